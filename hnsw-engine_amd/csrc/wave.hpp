@@ -1,0 +1,74 @@
+// Wavefront-level helpers for CDNA4 (wave64). One wavefront owns one query in the search
+// kernels, so "wave-uniform" values live in scalar registers and LDS traffic is ordered with
+// wave-scope fences instead of workgroup barriers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HM_DEV __device__ __forceinline__
+
+namespace hm {
+
+constexpr int kWave = 64;
+
+HM_DEV int lane_id() { return __lane_id(); }
+
+// Orders this wave's LDS/global accesses before and after the call (the rocPRIM pattern).
+HM_DEV void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+HM_DEV uint64_t ballot(bool p) { return __ballot(p); }
+
+HM_DEV int popc(uint64_t m) { return __popcll(m); }
+
+// Number of set bits of `m` below this lane.
+HM_DEV int rank_below(uint64_t m) {
+    uint32_t lo = (uint32_t)m, hi = (uint32_t)(m >> 32);
+    return __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+}
+
+HM_DEV int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+HM_DEV uint32_t uniform_u32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+HM_DEV uint64_t uniform_u64(uint64_t v) {
+    uint32_t lo = uniform_u32((uint32_t)v), hi = uniform_u32((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+HM_DEV int shfl(int v, int src) { return __shfl(v, src, kWave); }
+HM_DEV uint32_t shfl_u32(uint32_t v, int src) { return (uint32_t)__shfl((int)v, src, kWave); }
+HM_DEV uint64_t shfl_u64(uint64_t v, int src) {
+    uint32_t lo = shfl_u32((uint32_t)v, src), hi = shfl_u32((uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+HM_DEV int shfl_xor(int v, int m) { return __shfl_xor(v, m, kWave); }
+
+HM_DEV int wave_sum(int v) {
+    for (int m = 32; m >= 1; m >>= 1) v += shfl_xor(v, m);
+    return v;
+}
+
+HM_DEV int wave_max(int v) {
+    for (int m = 32; m >= 1; m >>= 1) v = max(v, shfl_xor(v, m));
+    return v;
+}
+
+// Minimum (key, id) pair over the wave; every lane receives it.
+HM_DEV void wave_min_pair(uint64_t &k, uint32_t &id) {
+    for (int m = 32; m >= 1; m >>= 1) {
+        uint32_t klo = (uint32_t)__shfl_xor((int)(uint32_t)k, m, kWave);
+        uint32_t khi = (uint32_t)__shfl_xor((int)(uint32_t)(k >> 32), m, kWave);
+        uint32_t oid = (uint32_t)__shfl_xor((int)id, m, kWave);
+        uint64_t ok = ((uint64_t)khi << 32) | klo;
+        if (ok < k || (ok == k && oid < id)) {
+            k = ok;
+            id = oid;
+        }
+    }
+}
+
+}  // namespace hm
