@@ -1,0 +1,450 @@
+// C ABI: index upload (the reference's attach_native) and the batch search entry points.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/hnsw_mi355x.h"
+#include "common.hpp"
+#include "index.hpp"
+#include "search.hpp"
+
+namespace hm {
+
+int search_slots(int mode, bool filt, int lds, int device_cus);
+hipError_t search_launch(int mode, bool filt, const SearchArgs &a, int blocks, hipStream_t s);
+
+static thread_local std::string g_error;
+
+void set_error(const std::string &msg) { g_error = msg; }
+
+int fail_hip(hipError_t e, const char *what) {
+    g_error = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? HM_ENOMEM : HM_EDEVICE;
+}
+
+int invalid(const std::string &msg) {
+    g_error = msg;
+    return HM_EINVAL;
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+namespace {
+
+// One batch already in HBM.
+struct Batch {
+    const float *q32 = nullptr;
+    const double *q64 = nullptr;
+    const int32_t *k = nullptr, *ef = nullptr, *midx = nullptr;
+    const uint32_t *mask_bits = nullptr;
+    int64_t mask_words = 0;
+    int k_all = 0, ef_all = 0, mask_all = -1, kstride = 0;
+    int max_ef = 0;
+    int64_t nq = 0;
+    int64_t *ids = nullptr;
+    double *dists = nullptr;
+    int32_t *dc = nullptr, *hops = nullptr, *fail = nullptr;
+};
+
+constexpr int64_t kDefaultSpill = 16384;
+
+// Capacities of the on-chip state for a batch whose largest max(ef, k) is max_ef.
+void plan(SearchArgs &a, int mode, const hm_index *h, int max_ef) {
+    max_ef = std::max(1, max_ef);
+    a.r_cap = (int)round_up(max_ef, 8);
+    a.c_cap = 1 << std::min(12, std::max(6, ceil_log2(2 * (int64_t)max_ef)));
+    a.h_log2 = std::min(12, std::max(10, ceil_log2((int64_t)max_ef * 64)));
+    a.lds_per_wave = lds_bytes_per_wave(mode, h->dim, h->dimp, a.r_cap, a.c_cap, a.h_log2);
+}
+
+int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, int max_slots,
+           hipStream_t s) {
+    SearchArgs a;
+    memset(&a, 0, sizeof(a));
+    a.links0 = h->links0.p;
+    a.links_up = h->links_up.p;
+    a.upper_off = h->upper.p;
+    a.n = h->n;
+    a.w0 = h->w0;
+    a.w1 = h->w1;
+    a.entry = (int)h->entry;
+    a.max_level = h->max_level;
+    a.vec_f32 = h->vec.p;
+    a.vec_u8 = mode == kExactU8 ? h->u8.p : h->codes.p;
+    a.dim = h->dim;
+    a.dimp = h->dimp;
+    a.metric = h->metric;
+    a.q_f32 = b.q32;
+    a.q_f64 = b.q64;
+    a.nq = (int)b.nq;
+    a.ef_q = b.ef;
+    a.k_q = b.k;
+    a.ef_all = b.ef_all;
+    a.k_all = b.k_all;
+    a.kstride = b.kstride;
+    a.mask_pool = b.mask_bits;
+    a.mask_words = b.mask_words;
+    a.mask_idx = b.midx;
+    a.mask_all = b.mask_all;
+    a.out_ids = b.ids;
+    a.out_d = b.dists;
+    a.out_dc = b.dc;
+    a.out_hops = b.hops;
+    a.out_fail = b.fail;
+    plan(a, mode, h, b.max_ef);
+    if (a.lds_per_wave > 160 * 1024) return invalid("ef too large for the on-chip search state");
+    int slots = search_slots(mode, filt, a.lds_per_wave, h->cus);
+    slots = (int)std::min<int64_t>(slots, std::max<int64_t>(1, b.nq));
+    if (max_slots > 0) slots = std::min(slots, max_slots);
+    a.slots = slots;
+    a.bitset_words = (h->n + 31) / 32;
+    HM_TRY(h->bits.ensure((size_t)slots * a.bitset_words, true, s), "bitset allocation");
+    a.bitsets = h->bits.p;
+    a.spill_cap = spill_cap;
+    HM_TRY(h->spill_key.ensure((size_t)slots * spill_cap), "spill allocation");
+    HM_TRY(h->spill_id.ensure((size_t)slots * spill_cap), "spill allocation");
+    a.spill_key = h->spill_key.p;
+    a.spill_id = h->spill_id.p;
+    HM_TRY(h->queue.ensure(1), "queue allocation");
+    a.queue = h->queue.p;
+    HM_TRY(search_launch(mode, filt, a, slots, s), "search launch");
+    return HM_OK;
+}
+
+// Packs boolean rows (n bytes each) into uint32 bitsets.
+std::vector<uint32_t> pack_masks(const uint8_t *pool, int64_t rows, int64_t n) {
+    const int64_t words = (n + 31) / 32;
+    std::vector<uint32_t> out((size_t)(rows * words), 0u);
+    for (int64_t r = 0; r < rows; ++r)
+        for (int64_t i = 0; i < n; ++i)
+            if (pool[r * n + i]) out[(size_t)(r * words + i / 32)] |= 1u << (i % 32);
+    return out;
+}
+
+int check_opts(const hm_index *h, const hm_search_opts *o) {
+    if (!o || o->nq < 0) return invalid("bad search options");
+    if (o->kstride < 1) return invalid("kstride must be >= 1");
+    if (!o->k && (o->k_all < 1 || o->k_all > o->kstride)) return invalid("k must be in [1, kstride]");
+    if (!o->ef && o->ef_all < 1) return invalid("ef must be >= 1");
+    if (h->n == 0 || h->entry < 0) return invalid("cannot search an empty index");
+    return HM_OK;
+}
+
+// Host-buffer search shared by hm_search and hm_search_codes.
+int search_host(hm_index *h, int mode, const float *q32, const double *q64,
+                const hm_search_opts *o, int64_t *ids, double *dists, int64_t *dcount,
+                int64_t *hops) {
+    int st = check_opts(h, o);
+    if (st) return st;
+    const int64_t nq = o->nq;
+    if (nq == 0) return HM_OK;
+    hipStream_t s = h->stream;
+    HM_TRY(hipSetDevice(h->device), "hipSetDevice");
+    Batch b;
+    b.nq = nq;
+    b.kstride = o->kstride;
+    b.k_all = o->k_all;
+    b.mask_all = o->mask_all;
+    // Per-query k and max(ef, k) (numba_engine.py:473,487).
+    std::vector<int32_t> kv, efv;
+    int max_ef = 0;
+    if (o->k || o->ef) {
+        kv.resize(nq);
+        efv.resize(nq);
+        for (int64_t i = 0; i < nq; ++i) {
+            int k = o->k ? o->k[i] : o->k_all;
+            int ef = o->ef ? o->ef[i] : o->ef_all;
+            if (k < 1 || k > o->kstride || ef < 1) return invalid("k or ef out of range");
+            kv[i] = k;
+            efv[i] = std::max(ef, k);
+            max_ef = std::max(max_ef, efv[i]);
+        }
+        HM_TRY(h->kq.ensure(nq), "alloc");
+        HM_TRY(h->efq.ensure(nq), "alloc");
+        HM_TRY(hipMemcpyAsync(h->kq.p, kv.data(), nq * 4, hipMemcpyHostToDevice, s), "copy k");
+        HM_TRY(hipMemcpyAsync(h->efq.p, efv.data(), nq * 4, hipMemcpyHostToDevice, s), "copy ef");
+        b.k = h->kq.p;
+        b.ef = h->efq.p;
+    } else {
+        b.ef_all = std::max(o->ef_all, o->k_all);
+        max_ef = b.ef_all;
+    }
+    b.max_ef = max_ef;
+    bool filt = false;
+    std::vector<uint32_t> bits;
+    if (o->mask_pool && o->n_masks > 0 && (o->mask_index || o->mask_all >= 0)) {
+        if (mode == kSeqU8) return invalid("the codes search takes no filter");
+        filt = true;
+        bits = pack_masks(o->mask_pool, o->n_masks, h->n);
+        b.mask_words = (h->n + 31) / 32;
+        HM_TRY(h->maskbits.ensure(bits.size()), "alloc");
+        HM_TRY(hipMemcpyAsync(h->maskbits.p, bits.data(), bits.size() * 4, hipMemcpyHostToDevice, s),
+               "copy masks");
+        b.mask_bits = h->maskbits.p;
+        if (o->mask_index) {
+            for (int64_t i = 0; i < nq; ++i)
+                if (o->mask_index[i] >= o->n_masks) return invalid("mask index out of range");
+            HM_TRY(h->midx.ensure(nq), "alloc");
+            HM_TRY(hipMemcpyAsync(h->midx.p, o->mask_index, nq * 4, hipMemcpyHostToDevice, s),
+                   "copy mask index");
+            b.midx = h->midx.p;
+        } else if (o->mask_all >= o->n_masks) {
+            return invalid("mask index out of range");
+        }
+    }
+    const size_t qn = (size_t)nq * h->dim;
+    if (q32) {
+        HM_TRY(h->q32.ensure(qn), "alloc");
+        HM_TRY(hipMemcpyAsync(h->q32.p, q32, qn * 4, hipMemcpyHostToDevice, s), "copy queries");
+        b.q32 = h->q32.p;
+    } else {
+        HM_TRY(h->q64.ensure(qn), "alloc");
+        HM_TRY(hipMemcpyAsync(h->q64.p, q64, qn * 8, hipMemcpyHostToDevice, s), "copy queries");
+        b.q64 = h->q64.p;
+    }
+    const size_t on = (size_t)nq * o->kstride;
+    HM_TRY(h->ids.ensure(on), "alloc");
+    HM_TRY(h->dists.ensure(on), "alloc");
+    HM_TRY(h->dc.ensure(nq), "alloc");
+    HM_TRY(h->hops.ensure(nq), "alloc");
+    HM_TRY(h->fail.ensure(nq), "alloc");
+    b.ids = h->ids.p;
+    b.dists = h->dists.p;
+    b.dc = h->dc.p;
+    b.hops = h->hops.p;
+    b.fail = h->fail.p;
+    st = launch(h, mode, filt, b, std::min<int64_t>(h->n, kDefaultSpill), 0, s);
+    if (st) return st;
+    std::vector<int32_t> failv(nq), dcv(nq), hv(nq);
+    HM_TRY(hipMemcpyAsync(failv.data(), h->fail.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
+    HM_TRY(hipStreamSynchronize(s), "search");
+    // Queries whose candidate spill overflowed run again with room for every node.
+    std::vector<int64_t> redo;
+    for (int64_t i = 0; i < nq; ++i)
+        if (failv[i]) redo.push_back(i);
+    if (!redo.empty() && h->n > kDefaultSpill) {
+        for (int64_t i : redo) {
+            Batch one = b;
+            one.nq = 1;
+            one.q32 = b.q32 ? b.q32 + (size_t)i * h->dim : nullptr;
+            one.q64 = b.q64 ? b.q64 + (size_t)i * h->dim : nullptr;
+            one.k = b.k ? b.k + i : nullptr;
+            one.ef = b.ef ? b.ef + i : nullptr;
+            one.midx = b.midx ? b.midx + i : nullptr;
+            one.ids = b.ids + (size_t)i * o->kstride;
+            one.dists = b.dists + (size_t)i * o->kstride;
+            one.dc = b.dc + i;
+            one.hops = b.hops + i;
+            one.fail = b.fail + i;
+            st = launch(h, mode, filt, one, h->n, 1, s);
+            if (st) return st;
+        }
+        HM_TRY(hipMemcpyAsync(failv.data(), h->fail.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
+        HM_TRY(hipStreamSynchronize(s), "search");
+        for (int64_t i = 0; i < nq; ++i)
+            if (failv[i]) return invalid("candidate spill overflow");
+    } else if (!redo.empty()) {
+        return invalid("candidate spill overflow");
+    }
+    HM_TRY(hipMemcpyAsync(ids, h->ids.p, on * 8, hipMemcpyDeviceToHost, s), "copy ids");
+    HM_TRY(hipMemcpyAsync(dists, h->dists.p, on * 8, hipMemcpyDeviceToHost, s), "copy dists");
+    HM_TRY(hipMemcpyAsync(dcv.data(), h->dc.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
+    HM_TRY(hipMemcpyAsync(hv.data(), h->hops.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
+    HM_TRY(hipStreamSynchronize(s), "search");
+    for (int64_t i = 0; i < nq; ++i) {
+        if (dcount) dcount[i] = dcv[i];
+        if (hops) hops[i] = hv[i];
+    }
+    return HM_OK;
+}
+
+bool all_u8_integers(const float *v, size_t count) {
+    for (size_t i = 0; i < count; ++i) {
+        float x = v[i];
+        if (!(x >= 0.0f && x <= 255.0f) || x != floorf(x)) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hm_open(const hm_native_graph *g, int device, hm_index **out) {
+    if (!g || !out) return invalid("null argument");
+    *out = nullptr;
+    if (g->n < 0 || g->dim < 1 || g->M < 2 || g->metric < 0 || g->metric > 2)
+        return invalid("bad graph header");
+    if (g->n >= (int64_t(1) << 31) - 1) return invalid("more than 2^31-2 nodes");
+    HM_TRY(hipSetDevice(device), "hipSetDevice");
+    hm_index *h = new hm_index();
+    h->device = device;
+    h->n = g->n;
+    h->rows = g->rows;
+    h->entry = g->entry_point;
+    h->dim = g->dim;
+    h->M = g->M;
+    h->metric = g->metric;
+    h->max_level = g->max_level;
+    const int m0 = 2 * g->M;
+    h->w0 = (int)round_up(m0, 16);
+    h->w1 = (int)round_up(g->M, 16);
+    h->dimp = (int)round_up(g->dim, 16);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->cus = prop.multiProcessorCount;
+    auto fail = [&](int st) {
+        hm_close(h);
+        return st;
+    };
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(fail_hip(e, "stream"));
+    const int64_t n = g->n;
+    // Layer-0 rows: the used slots in stored order, repeats dropped, -1 beyond.
+    {
+        std::vector<int32_t> rows((size_t)n * h->w0, -1);
+        for (int64_t i = 0; i < n; ++i) {
+            const int32_t *src = g->links0 + (size_t)i * (m0 + 1);
+            int32_t *dst = rows.data() + (size_t)i * h->w0;
+            int c = std::min(g->count0[i], h->w0), w = 0;
+            for (int j = 0; j < c; ++j) {
+                bool rep = false;
+                for (int t = 0; t < w; ++t) rep |= dst[t] == src[j];
+                if (!rep) dst[w++] = src[j];
+            }
+        }
+        if ((e = h->links0.ensure(rows.size())) != hipSuccess) return fail(fail_hip(e, "alloc"));
+        e = hipMemcpy(h->links0.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(fail_hip(e, "upload links0"));
+    }
+    {
+        std::vector<int32_t> rows((size_t)g->rows * h->w1, -1);
+        for (int64_t r = 0; r < g->rows; ++r) {
+            const int32_t *src = g->links_up + (size_t)r * (g->M + 1);
+            int32_t *dst = rows.data() + (size_t)r * h->w1;
+            int c = std::min(g->count_up[r], h->w1), w = 0;
+            for (int j = 0; j < c; ++j) {
+                bool rep = false;
+                for (int t = 0; t < w; ++t) rep |= dst[t] == src[j];
+                if (!rep) dst[w++] = src[j];
+            }
+        }
+        std::vector<int32_t> up((size_t)n);
+        for (int64_t i = 0; i < n; ++i) up[i] = (int32_t)g->upper_offset[i];
+        if ((e = h->links_up.ensure(rows.size())) != hipSuccess) return fail(fail_hip(e, "alloc"));
+        if (!rows.empty()) {
+            e = hipMemcpy(h->links_up.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice);
+            if (e != hipSuccess) return fail(fail_hip(e, "upload links_up"));
+        }
+        if ((e = h->upper.ensure(up.size())) != hipSuccess) return fail(fail_hip(e, "alloc"));
+        if (n) {
+            e = hipMemcpy(h->upper.p, up.data(), up.size() * 4, hipMemcpyHostToDevice);
+            if (e != hipSuccess) return fail(fail_hip(e, "upload upper_offset"));
+        }
+    }
+    const size_t vn = (size_t)n * g->dim;
+    if ((e = h->vec.ensure(vn)) != hipSuccess) return fail(fail_hip(e, "alloc"));
+    if (vn) {
+        e = hipMemcpy(h->vec.p, g->vectors, vn * 4, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(fail_hip(e, "upload vectors"));
+    }
+    // Exact 8-bit copy when every value is an integer in 0..255 (zero padded to 16 bytes).
+    if (g->metric != HM_COSINE && n > 0 && all_u8_integers(g->vectors, vn)) {
+        std::vector<uint8_t> u((size_t)n * h->dimp, 0);
+        for (int64_t i = 0; i < n; ++i)
+            for (int j = 0; j < g->dim; ++j)
+                u[(size_t)i * h->dimp + j] = (uint8_t)g->vectors[(size_t)i * g->dim + j];
+        if ((e = h->u8.ensure(u.size())) != hipSuccess) return fail(fail_hip(e, "alloc"));
+        e = hipMemcpy(h->u8.p, u.data(), u.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(fail_hip(e, "upload u8"));
+        h->exact_u8 = true;
+    }
+    *out = h;
+    return HM_OK;
+}
+
+void hm_close(hm_index *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->release_all();
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int hm_has_exact_u8(const hm_index *h) { return h && h->exact_u8 ? 1 : 0; }
+
+int hm_queries_exact(const float *q, int64_t count) {
+    return q && count >= 0 && all_u8_integers(q, (size_t)count) ? 1 : 0;
+}
+
+int hm_search(hm_index *h, const float *queries, const hm_search_opts *o, int64_t *ids,
+              double *dists, int64_t *dcount, int64_t *hops) {
+    if (!h || !queries || !o || !ids || !dists) return invalid("null argument");
+    const bool exact = h->exact_u8 && hm_queries_exact(queries, o->nq * h->dim);
+    return search_host(h, exact ? kExactU8 : kSeqF32, queries, nullptr, o, ids, dists, dcount,
+                       hops);
+}
+
+int hm_search_dev(hm_index *h, const float *d_queries, const hm_search_opts *o,
+                  const uint32_t *d_mask_bits, int exact_u8, int64_t *d_ids, double *d_dists,
+                  int32_t *d_dcount, int32_t *d_hops, int32_t *d_fail, void *stream) {
+    if (!h || !d_queries || !o || !d_ids || !d_dists) return invalid("null argument");
+    int st = check_opts(h, o);
+    if (st) return st;
+    if (exact_u8 && !h->exact_u8) return invalid("the index keeps no exact 8-bit copy");
+    if (o->k || o->ef) return invalid("hm_search_dev takes uniform k and ef");
+    if (o->nq == 0) return HM_OK;
+    HM_TRY(hipSetDevice(h->device), "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    Batch b;
+    b.nq = o->nq;
+    b.q32 = d_queries;
+    b.kstride = o->kstride;
+    b.k_all = o->k_all;
+    b.ef_all = std::max(o->ef_all, o->k_all);
+    b.max_ef = b.ef_all;
+    b.mask_bits = d_mask_bits;
+    b.mask_words = (h->n + 31) / 32;
+    b.midx = o->mask_index;
+    b.mask_all = o->mask_all;
+    const bool filt = d_mask_bits != nullptr && (o->mask_index || o->mask_all >= 0);
+    HM_TRY(h->dc.ensure(o->nq), "alloc");
+    HM_TRY(h->hops.ensure(o->nq), "alloc");
+    HM_TRY(h->fail.ensure(o->nq), "alloc");
+    b.ids = d_ids;
+    b.dists = d_dists;
+    b.dc = d_dcount ? d_dcount : h->dc.p;
+    b.hops = d_hops ? d_hops : h->hops.p;
+    b.fail = d_fail ? d_fail : h->fail.p;
+    return launch(h, exact_u8 ? kExactU8 : kSeqF32, filt, b, std::min<int64_t>(h->n, kDefaultSpill),
+                  0, s);
+}
+
+int hm_attach_codes(hm_index *h, const uint8_t *codes) {
+    if (!h || !codes) return invalid("null argument");
+    HM_TRY(hipSetDevice(h->device), "hipSetDevice");
+    std::vector<uint8_t> u((size_t)h->n * h->dimp, 0);
+    for (int64_t i = 0; i < h->n; ++i)
+        memcpy(u.data() + (size_t)i * h->dimp, codes + (size_t)i * h->dim, h->dim);
+    HM_TRY(h->codes.ensure(u.size()), "alloc");
+    HM_TRY(hipMemcpy(h->codes.p, u.data(), u.size(), hipMemcpyHostToDevice), "upload codes");
+    h->has_codes = true;
+    return HM_OK;
+}
+
+int hm_search_codes(hm_index *h, const double *queries, const hm_search_opts *o, int64_t *ids,
+                    double *dists, int64_t *dcount, int64_t *hops) {
+    if (!h || !queries || !o || !ids || !dists) return invalid("null argument");
+    if (!h->has_codes) return invalid("no codes attached");
+    return search_host(h, kSeqU8, nullptr, queries, o, ids, dists, dcount, hops);
+}
+
+const char *hm_last_error(void) { return g_error.c_str(); }
+
+}  // extern "C"

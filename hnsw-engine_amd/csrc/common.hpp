@@ -1,0 +1,53 @@
+// Host-side helpers shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace hm {
+
+void set_error(const std::string &msg);
+int fail_hip(hipError_t e, const char *what);
+
+#define HM_TRY(expr, what)                            \
+    do {                                              \
+        hipError_t _e = (expr);                       \
+        if (_e != hipSuccess) return fail_hip(_e, what); \
+    } while (0)
+
+// Grow-only device buffer.
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    // Reallocates when `count` exceeds the capacity; a new allocation is zero-filled when
+    // `zero` is set (contents are not preserved).
+    hipError_t ensure(size_t count, bool zero = false, hipStream_t s = nullptr) {
+        if (count <= cap && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = count ? count : 1;
+        hipError_t e = hipMalloc((void **)&p, want * sizeof(T));
+        if (e != hipSuccess) return e;
+        cap = want;
+        if (zero) e = hipMemsetAsync(p, 0, want * sizeof(T), s);
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+inline int ceil_log2(int64_t v) {
+    int r = 0;
+    while ((int64_t(1) << r) < v) ++r;
+    return r;
+}
+
+inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+}  // namespace hm

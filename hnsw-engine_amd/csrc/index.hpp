@@ -1,0 +1,54 @@
+// The index handle behind the C ABI: the graph in HBM plus scratch kept between calls.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+struct hm_index {
+    int device = 0;
+    int64_t n = 0, rows = 0, entry = -1;
+    int dim = 0, M = 0, metric = 0, max_level = -1;
+    int w0 = 0, w1 = 0, dimp = 0;  // repacked row widths, padded dimension of uint8 rows
+    int cus = 1;
+    bool exact_u8 = false, has_codes = false;
+    hipStream_t stream = nullptr;
+    // graph: links0 (n, w0), links_up (rows, w1), upper (n,) first upper row, -1 beyond counts
+    hm::DevBuf<int32_t> links0, links_up, upper;
+    hm::DevBuf<float> vec;        // (n, dim)
+    hm::DevBuf<uint8_t> u8, codes;  // (n, dimp)
+    // search scratch
+    hm::DevBuf<uint32_t> bits;
+    hm::DevBuf<uint64_t> spill_key;
+    hm::DevBuf<uint32_t> spill_id;
+    hm::DevBuf<int32_t> queue;
+    // staging of the host-buffer entry points
+    hm::DevBuf<float> q32;
+    hm::DevBuf<double> q64;
+    hm::DevBuf<int32_t> kq, efq, midx, dc, hops, fail;
+    hm::DevBuf<uint32_t> maskbits;
+    hm::DevBuf<int64_t> ids, cand;
+    hm::DevBuf<double> dists;
+
+    void release_all() {
+        hm::DevBuf<int32_t> *i32[] = {&links0, &links_up, &upper, &queue, &kq,
+                                      &efq,    &midx,     &dc,    &hops,  &fail};
+        for (auto *b : i32) b->release();
+        vec.release();
+        u8.release();
+        codes.release();
+        bits.release();
+        spill_key.release();
+        spill_id.release();
+        q32.release();
+        q64.release();
+        maskbits.release();
+        ids.release();
+        cand.release();
+        dists.release();
+    }
+};
+
+namespace hm {
+int invalid(const std::string &msg);
+}

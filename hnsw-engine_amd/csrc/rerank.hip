@@ -1,0 +1,114 @@
+// Exact re-ranking (numba_engine.py:506-530, _rerank_batch): each query's candidate ids (-1
+// skipped) are re-scored with the sequential float64 distance to the float32 vectors and the
+// best k by (distance, id) are kept. One wave per query; the scored list is sorted in LDS.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/hnsw_mi355x.h"
+#include "index.hpp"
+#include "keys.hpp"
+#include "wave.hpp"
+
+namespace hm {
+
+// numba_engine.py:57-70 in its exact order (float64, left to right, unfused).
+HM_DEV double seq_distance(const float *row, const double *q, int dim, int metric) {
+#pragma clang fp contract(off)
+    double acc = 0.0;
+    if (metric == kL2) {
+        for (int i = 0; i < dim; ++i) {
+            double d = (double)row[i] - q[i];
+            acc = acc + d * d;
+        }
+        return acc;
+    }
+    for (int i = 0; i < dim; ++i) acc = acc + (double)row[i] * q[i];
+    return metric == kIP ? -acc : 1.0 - acc;
+}
+
+__global__ __launch_bounds__(64) void rerank_kernel(const float *vec, int dim, int metric,
+                                                    const float *queries, const int64_t *cand,
+                                                    int fetch, int k, int p2, int64_t *out_ids,
+                                                    double *out_d) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+    uint64_t *keys = (uint64_t *)lds;
+    uint32_t *ids = (uint32_t *)(lds + (size_t)p2 * 8);
+    double *q = (double *)(lds + (size_t)p2 * 12 + ((16 - (p2 * 12) % 16) % 16));
+    const int lane = lane_id();
+    const int qi = blockIdx.x;
+    for (int i = lane; i < dim; i += 64) q[i] = (double)queries[(size_t)qi * dim + i];
+    wave_sync();
+    int valid = 0;
+    for (int j = lane; j < fetch; j += 64) {
+        const int64_t e = cand[(size_t)qi * fetch + j];
+        if (e >= 0) valid += 1;
+    }
+    valid = wave_sum(valid);
+    // Compact the valid ids in candidate order, then score them.
+    int w = 0;
+    for (int b = 0; b < fetch; b += 64) {
+        const int j = b + lane;
+        const int64_t e = j < fetch ? cand[(size_t)qi * fetch + j] : -1;
+        const uint64_t m = ballot(e >= 0);
+        if (e >= 0) {
+            const int d = w + rank_below(m);
+            ids[d] = (uint32_t)e;
+            keys[d] = key_of_double(seq_distance(vec + (size_t)e * dim, q, dim, metric));
+        }
+        w += popc(m);
+    }
+    wave_sync();
+    bitonic_sort(keys, ids, valid, p2);
+    for (int i = lane; i < k; i += 64) {
+        if (i < valid) {
+            out_ids[(size_t)qi * k + i] = ids[i];
+            out_d[(size_t)qi * k + i] = double_of_key(keys[i], metric);
+        } else {
+            out_ids[(size_t)qi * k + i] = -1;
+            out_d[(size_t)qi * k + i] = __builtin_huge_val();
+        }
+    }
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+extern "C" int hm_rerank(hm_index *h, const float *queries, int64_t nq, const int64_t *candidates,
+                         int32_t fetch, int32_t k, int64_t *ids, double *dists) {
+    if (!h || !queries || !candidates || !ids || !dists) return invalid("null argument");
+    if (nq < 0 || fetch < 0 || k < 1) return invalid("bad rerank sizes");
+    if (nq == 0) return HM_OK;
+    for (int64_t i = 0; i < nq * fetch; ++i)
+        if (candidates[i] < -1 || candidates[i] >= h->n)
+            return invalid("candidate ids outside [-1, n)");
+    int p2 = 1;
+    while (p2 < fetch) p2 <<= 1;
+    const int lds = p2 * 12 + 16 + h->dim * 8;
+    if (lds > 160 * 1024) return invalid("rerank fetch too large");
+    hipStream_t s = h->stream;
+    HM_TRY(hipSetDevice(h->device), "hipSetDevice");
+    if (lds > 64 * 1024)
+        HM_TRY(hipFuncSetAttribute((const void *)rerank_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds),
+               "attribute");
+    HM_TRY(h->q32.ensure((size_t)nq * h->dim), "alloc");
+    HM_TRY(h->cand.ensure((size_t)nq * std::max(fetch, 1)), "alloc");
+    HM_TRY(h->ids.ensure((size_t)nq * k), "alloc");
+    HM_TRY(h->dists.ensure((size_t)nq * k), "alloc");
+    HM_TRY(hipMemcpyAsync(h->q32.p, queries, (size_t)nq * h->dim * 4, hipMemcpyHostToDevice, s),
+           "copy");
+    if (fetch)
+        HM_TRY(hipMemcpyAsync(h->cand.p, candidates, (size_t)nq * fetch * 8, hipMemcpyHostToDevice,
+                              s),
+               "copy");
+    hipLaunchKernelGGL(rerank_kernel, dim3((unsigned)nq), dim3(64), lds, s, h->vec.p, h->dim,
+                       h->metric, h->q32.p, h->cand.p, fetch, k, p2, h->ids.p, h->dists.p);
+    HM_TRY(hipGetLastError(), "rerank launch");
+    HM_TRY(hipMemcpyAsync(ids, h->ids.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, s), "copy");
+    HM_TRY(hipMemcpyAsync(dists, h->dists.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, s), "copy");
+    HM_TRY(hipStreamSynchronize(s), "rerank");
+    return HM_OK;
+}

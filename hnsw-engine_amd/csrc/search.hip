@@ -1,0 +1,805 @@
+// Batched HNSW search on CDNA4: one wavefront owns one query (Algorithm 5 of the reference,
+// numba_engine.py:422-503), fed by a persistent work queue.
+//
+// Exactness. The reference pops candidates in (distance, id) order, scans a node's links in
+// stored order and accepts neighbour j iff |R| < ef or d_j < worst(R) (numba_engine.py:126-145),
+// updating R after every accept. For one expansion that serial rule has a closed parallel form:
+//
+//     accept_j  <=>  #{r in R : d_r <= d_j} + #{i < j new : d_i <= d_j} < ef
+//
+// (an earlier rejected i with d_i <= d_j forces j's rejection anyway, so counting all earlier
+// new neighbours is equivalent). R after the expansion is the ef smallest (d, id) of R and the
+// accepted set, which is what evicting the largest tuple after each push yields. The filtered
+// variant (numba_engine.py:151-198) counts only allowed i and merges only allowed accepts into R;
+// its break needs |R| == ef. Links rows wider than 64 are processed in order, 64 at a time, with
+// R and C updated between chunks, which is again the serial order.
+//
+// State of a query (per wave, in LDS): R (sorted (key, id), <= ef), C (sorted, the candidates
+// not yet expanded, capacity c_cap, overflow spilled to global memory with a radix-select
+// refill), a visited hash (overflow: a per-wave global bitset), staging for one expansion.
+// Candidates with d > worst(R) once R is full can never be expanded (the first such pop ends the
+// search), so they are dropped at merge time; d == worst is kept.
+#include "search.hpp"
+#include "wave.hpp"
+#include "keys.hpp"
+
+namespace hm {
+
+// ---------------------------------------------------------------------------- LDS layout
+
+struct Lds {
+    uint8_t *qb;    // exact: dimp query bytes; sequential: dim doubles
+    uint64_t *rk[2];
+    uint32_t *ri[2];
+    uint64_t *ck[2];
+    uint32_t *ci[2];
+    uint64_t *sak, *srk;  // accepted (all / allowed) of one expansion, sorted
+    uint32_t *sai, *sri;
+    uint64_t *nbk;  // new neighbours of one expansion, in link order
+    uint32_t *nbi;
+    int32_t *nbacc;
+    uint32_t *nbal;
+    uint32_t *hash;
+    uint32_t *hist;  // 256 bins for the refill's radix select
+    uint64_t *misc;  // [0..1]: first spilled entry
+};
+
+HM_DEV Lds carve(uint8_t *base, const SearchArgs &a, int mode) {
+    auto up = [](int v) { return (v + 15) & ~15; };
+    Lds L;
+    uint8_t *p = base;
+    L.qb = p;
+    p += mode == kExactU8 ? up(a.dimp) : up(a.dim * 8);
+    for (int b = 0; b < 2; ++b) {
+        L.rk[b] = (uint64_t *)p;
+        p += up(a.r_cap * 8);
+        L.ri[b] = (uint32_t *)p;
+        p += up(a.r_cap * 4);
+    }
+    for (int b = 0; b < 2; ++b) {
+        L.ck[b] = (uint64_t *)p;
+        p += up(a.c_cap * 8);
+        L.ci[b] = (uint32_t *)p;
+        p += up(a.c_cap * 4);
+    }
+    L.sak = (uint64_t *)p; p += 64 * 8;
+    L.sai = (uint32_t *)p; p += 64 * 4;
+    L.srk = (uint64_t *)p; p += 64 * 8;
+    L.sri = (uint32_t *)p; p += 64 * 4;
+    L.nbk = (uint64_t *)p; p += 64 * 8;
+    L.nbi = (uint32_t *)p; p += 64 * 4;
+    L.nbacc = (int32_t *)p; p += 64 * 4;
+    L.nbal = (uint32_t *)p; p += 64 * 4;
+    L.hash = (uint32_t *)p; p += 4 << a.h_log2;
+    L.hist = (uint32_t *)p; p += 256 * 4;
+    L.misc = (uint64_t *)p;
+    return L;
+}
+
+// Number of entries of sorted (k, i)[0..n) strictly below (bk, bi).
+HM_DEV int lower_count(const uint64_t *k, const uint32_t *id, int n, uint64_t bk, uint32_t bi) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (less(k[mid], id[mid], bk, bi)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Number of entries of sorted k[0..n) with key <= bk (distance-only comparison).
+HM_DEV int upper_count_key(const uint64_t *k, int n, uint64_t bk) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (k[mid] <= bk) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------- distances
+
+HM_DEV int32_t exact_partial(uint4 x, uint4 q, int metric) {
+    uint32_t xq = 0;
+    xq = __builtin_amdgcn_udot4(x.x, q.x, xq, false);
+    xq = __builtin_amdgcn_udot4(x.y, q.y, xq, false);
+    xq = __builtin_amdgcn_udot4(x.z, q.z, xq, false);
+    xq = __builtin_amdgcn_udot4(x.w, q.w, xq, false);
+    if (metric != kL2) return (int32_t)xq;
+    uint32_t xx = 0;
+    xx = __builtin_amdgcn_udot4(x.x, x.x, xx, false);
+    xx = __builtin_amdgcn_udot4(x.y, x.y, xx, false);
+    xx = __builtin_amdgcn_udot4(x.z, x.z, xx, false);
+    xx = __builtin_amdgcn_udot4(x.w, x.w, xx, false);
+    return (int32_t)xx - 2 * (int32_t)xq;
+}
+
+// Exact integer distances of the nnew ids in L.nbi to the 8-bit query. Lanes 0..nnew-1 get
+// their key. Every partial sum is an exact integer, so any summation order equals the
+// reference's float64 sum.
+HM_DEV uint64_t dist_exact(const SearchArgs &a, const Lds &L, int nnew, int32_t qq) {
+    const int lane = lane_id();
+    const int cpv = a.dimp >> 4;
+    const uint4 *qv = (const uint4 *)L.qb;
+    if ((cpv & (cpv - 1)) == 0 && cpv <= 64) {
+        const int lg = __builtin_ctz(cpv);
+        const int c = lane & (cpv - 1);
+        const uint4 q4 = qv[c];
+        const int vpi = 64 >> lg;
+        constexpr int kBatch = 8;
+        for (int v0 = 0; v0 < nnew; v0 += vpi * kBatch) {
+            uint4 xs[kBatch];
+#pragma unroll
+            for (int it = 0; it < kBatch; ++it) {
+                int v = v0 + it * vpi + (lane >> lg);
+                if (v0 + it * vpi < nnew && v < nnew) {
+                    uint32_t id = L.nbi[v];
+                    xs[it] = *(const uint4 *)(a.vec_u8 + (size_t)id * a.dimp + (size_t)c * 16);
+                }
+            }
+#pragma unroll
+            for (int it = 0; it < kBatch; ++it) {
+                if (v0 + it * vpi >= nnew) break;
+                int v = v0 + it * vpi + (lane >> lg);
+                int32_t part = v < nnew ? exact_partial(xs[it], q4, a.metric) : 0;
+                for (int m = 1; m < cpv; m <<= 1) part += shfl_xor(part, m);
+                if (c == 0 && v < nnew) L.nbacc[v] = part;
+            }
+        }
+    } else {
+        if (lane < nnew) L.nbacc[lane] = 0;
+        wave_sync();
+        const int total = nnew * cpv;
+        for (int t = lane; t < total; t += 64) {
+            int v = t / cpv, c = t - v * cpv;
+            uint32_t id = L.nbi[v];
+            uint4 x = *(const uint4 *)(a.vec_u8 + (size_t)id * a.dimp + (size_t)c * 16);
+            atomicAdd(&L.nbacc[v], exact_partial(x, qv[c], a.metric));
+        }
+    }
+    wave_sync();
+    uint64_t key = kInfKey;
+    if (lane < nnew) {
+        int32_t s = L.nbacc[lane];
+        key = a.metric == kL2 ? key_of_exact(s + qq, kL2) : key_of_exact(-s, a.metric);
+    }
+    return key;
+}
+
+// numba_engine.py:57-70 in its exact operation order: one lane per neighbour, float64,
+// left to right, no fused multiply-add.
+template <bool U8>
+HM_DEV uint64_t dist_seq(const SearchArgs &a, const Lds &L, int nnew) {
+#pragma clang fp contract(off)
+    const int lane = lane_id();
+    if (lane >= nnew) return kInfKey;
+    const double *q = (const double *)L.qb;
+    const uint32_t id = L.nbi[lane];
+    const int dim = a.dim;
+    double acc = 0.0;
+    if (!U8) {
+        const float *row = a.vec_f32 + (size_t)id * dim;
+        if ((dim & 3) == 0) {
+            const float4 *r4 = (const float4 *)row;
+            if (a.metric == kL2) {
+                for (int i = 0; i < dim; i += 4) {
+                    float4 x = r4[i >> 2];
+                    double d0 = (double)x.x - q[i];
+                    acc = acc + d0 * d0;
+                    double d1 = (double)x.y - q[i + 1];
+                    acc = acc + d1 * d1;
+                    double d2 = (double)x.z - q[i + 2];
+                    acc = acc + d2 * d2;
+                    double d3 = (double)x.w - q[i + 3];
+                    acc = acc + d3 * d3;
+                }
+            } else {
+                for (int i = 0; i < dim; i += 4) {
+                    float4 x = r4[i >> 2];
+                    acc = acc + (double)x.x * q[i];
+                    acc = acc + (double)x.y * q[i + 1];
+                    acc = acc + (double)x.z * q[i + 2];
+                    acc = acc + (double)x.w * q[i + 3];
+                }
+            }
+        } else if (a.metric == kL2) {
+            for (int i = 0; i < dim; ++i) {
+                double d = (double)row[i] - q[i];
+                acc = acc + d * d;
+            }
+        } else {
+            for (int i = 0; i < dim; ++i) acc = acc + (double)row[i] * q[i];
+        }
+    } else {
+        const uint8_t *row = a.vec_u8 + (size_t)id * a.dimp;
+        for (int i0 = 0; i0 < dim; i0 += 16) {
+            uint4 w = *(const uint4 *)(row + i0);
+            uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+            const int lim = dim - i0 < 16 ? dim - i0 : 16;
+            for (int i = 0; i < lim; ++i) {
+                double x = (double)((ws[i >> 2] >> (8 * (i & 3))) & 255u);
+                if (a.metric == kL2) {
+                    double d = x - q[i0 + i];
+                    acc = acc + d * d;
+                } else {
+                    acc = acc + x * q[i0 + i];
+                }
+            }
+        }
+    }
+    double d = a.metric == kL2 ? acc : (a.metric == kIP ? -acc : 1.0 - acc);
+    return key_of_double(d);
+}
+
+template <int MODE>
+HM_DEV uint64_t distances(const SearchArgs &a, const Lds &L, int nnew, int32_t qq) {
+    if constexpr (MODE == kExactU8) return dist_exact(a, L, nnew, qq);
+    else return dist_seq<MODE == kSeqU8>(a, L, nnew);
+}
+
+template <int MODE>
+HM_DEV double decode(uint64_t k, int metric) {
+    if constexpr (MODE == kExactU8) return double_of_exact(k, metric);
+    else return double_of_key(k, metric);
+}
+
+// ---------------------------------------------------------------------------- visited set
+
+struct Visited {
+    uint32_t *bits;
+    int hlog;    // active hash size is 1 << hlog
+    int count;   // ids inserted in this layer
+    bool use_bits, dirty;
+};
+
+HM_DEV void visited_reset(const SearchArgs &a, const Lds &L, Visited &V, int hlog) {
+    const int lane = lane_id();
+    if (V.dirty) {
+        for (int64_t w = lane; w < a.bitset_words; w += 64) V.bits[w] = 0u;
+        __threadfence();
+        V.dirty = false;
+    }
+    V.use_bits = false;
+    V.hlog = hlog;
+    V.count = 0;
+    for (int i = lane; i < (1 << hlog); i += 64) L.hash[i] = 0u;
+    wave_sync();
+}
+
+// Moves every hashed id into the bitset once the hash is half full.
+HM_DEV void visited_make_room(const Lds &L, Visited &V, int incoming) {
+    if (V.use_bits || V.count + incoming <= (1 << V.hlog) / 2) return;
+    const int lane = lane_id();
+    for (int i = lane; i < (1 << V.hlog); i += 64) {
+        uint32_t v = L.hash[i];
+        if (v) atomicOr(&V.bits[(v - 1) >> 5], 1u << ((v - 1) & 31));
+    }
+    __threadfence();
+    V.use_bits = true;
+    V.dirty = true;
+}
+
+// True if `id` was not yet visited in this layer; marks it.
+HM_DEV bool visit(const Lds &L, const Visited &V, uint32_t id) {
+    if (V.use_bits) {
+        uint32_t bit = 1u << (id & 31);
+        return (atomicOr(&V.bits[id >> 5], bit) & bit) == 0u;
+    }
+    const uint32_t v = id + 1u, mask = (1u << V.hlog) - 1u;
+    uint32_t pos = (id * 2654435761u) >> (32 - V.hlog);
+    for (;;) {
+        uint32_t old = atomicCAS(&L.hash[pos], 0u, v);
+        if (old == 0u) return true;
+        if (old == v) return false;
+        pos = (pos + 1u) & mask;
+    }
+}
+
+// ---------------------------------------------------------------------------- query state
+
+struct Query {
+    int ef, k;
+    const uint32_t *mask;  // null: every node allowed
+    int32_t qq;
+    int dcount, hops;
+};
+
+HM_DEV bool allowed(const Query &Q, uint32_t id) {
+    return Q.mask == nullptr || ((Q.mask[id >> 5] >> (id & 31)) & 1u);
+}
+
+// Greedy ef = 1 search on an upper layer (numba_engine.py:98-148 with ef = 1): the result is
+// the single best node; a neighbour is accepted iff strictly closer than the current best, so
+// the candidate after an expansion is the first occurrence of the row's minimum when that beats
+// the current best, and the search ends after the first expansion that does not improve.
+template <int MODE>
+HM_DEV void greedy_layer(const SearchArgs &a, const Lds &L, Visited &V, Query &Q, int layer,
+                         uint64_t &cur_k, uint32_t &cur_id) {
+    const int lane = lane_id();
+    visited_reset(a, L, V, a.h_log2 < 9 ? a.h_log2 : 9);
+    if (lane == 0) visit(L, V, cur_id);
+    V.count = 1;
+    wave_sync();
+    for (;;) {
+        Q.hops += 1;
+        const int64_t row = (int64_t)a.upper_off[cur_id] + layer - 1;
+        bool improved = false;
+        for (int base = 0; base < a.w1; base += 64) {
+            const int j = base + lane;
+            const int32_t nb = j < a.w1 ? a.links_up[row * a.w1 + j] : -1;
+            visited_make_room(L, V, 64);
+            const bool isnew = nb >= 0 && visit(L, V, (uint32_t)nb);
+            const uint64_t m = ballot(isnew);
+            const int nnew = popc(m);
+            if (nnew == 0) continue;
+            if (isnew) L.nbi[rank_below(m)] = (uint32_t)nb;
+            wave_sync();
+            V.count += nnew;
+            uint64_t key = distances<MODE>(a, L, nnew, Q.qq);
+            Q.dcount += nnew;
+            uint32_t idx = lane < nnew ? (uint32_t)lane : kInfId;
+            wave_min_pair(key, idx);
+            if (key < cur_k) {
+                cur_k = key;
+                cur_id = L.nbi[idx];
+                improved = true;
+            }
+            wave_sync();
+        }
+        if (!improved) break;
+    }
+}
+
+// Selects a prefix of the spilled candidates (by (key, id)) of size 1..c_cap: returns the
+// boundary (bk, bi) such that the entries strictly below it form that prefix. Radix select over
+// the 96-bit composite (key, id), 8 bits at a time, most significant first.
+HM_DEV void spill_boundary(const Lds &L, const uint64_t *sk, const uint32_t *si, int w, int cap,
+                           uint64_t &bk, uint32_t &bi) {
+    const int lane = lane_id();
+    uint64_t pk = 0;
+    uint32_t pi = 0;
+    for (int t = 0; t < 12; ++t) {
+        for (int i = lane; i < 256; i += 64) L.hist[i] = 0u;
+        wave_sync();
+        for (int b = 0; b < w; b += 64) {
+            const int j = b + lane;
+            if (j >= w) continue;
+            const uint64_t k = sk[j];
+            const uint32_t id = si[j];
+            bool match;
+            if (t == 0) match = true;
+            else if (t < 8) match = (k >> (64 - 8 * t)) == (pk >> (64 - 8 * t));
+            else if (t == 8) match = k == pk;
+            else match = k == pk && (id >> (32 - 8 * (t - 8))) == (pi >> (32 - 8 * (t - 8)));
+            if (!match) continue;
+            const uint32_t d = t < 8 ? (uint32_t)(k >> (56 - 8 * t)) & 255u
+                                     : (id >> (24 - 8 * (t - 8))) & 255u;
+            atomicAdd(&L.hist[d], 1u);
+        }
+        wave_sync();
+        // Inclusive scan of the 256 bins, 4 per lane.
+        const uint32_t h0 = L.hist[4 * lane], h1 = L.hist[4 * lane + 1];
+        const uint32_t h2 = L.hist[4 * lane + 2], h3 = L.hist[4 * lane + 3];
+        int s = (int)(h0 + h1 + h2 + h3);
+        int incl = s;
+        for (int o = 1; o < 64; o <<= 1) {
+            int v = shfl(incl, lane >= o ? lane - o : lane);
+            if (lane >= o) incl += v;
+        }
+        const int excl = incl - s;
+        // First bin whose inclusive count exceeds the capacity.
+        int dstar = 256, before = 0;
+        if (incl > cap && excl <= cap) {
+            int c0 = excl;
+            const uint32_t hs[4] = {h0, h1, h2, h3};
+            for (int q = 0; q < 4; ++q) {
+                if (c0 + (int)hs[q] > cap) {
+                    dstar = 4 * lane + q;
+                    before = c0;
+                    break;
+                }
+                c0 += (int)hs[q];
+            }
+        }
+        const uint64_t found = ballot(dstar < 256);
+        const int src = __ffsll((long long)found) - 1;
+        dstar = shfl(dstar, src);
+        before = shfl(before, src);
+        if (t < 8) pk |= (uint64_t)dstar << (56 - 8 * t);
+        else pi |= (uint32_t)dstar << (24 - 8 * (t - 8));
+        if (before >= 1) break;  // entries below the boundary: 1..cap of them
+        wave_sync();
+    }
+    bk = pk;
+    bi = pi;
+}
+
+// ---------------------------------------------------------------------------- layer 0
+
+template <int MODE, bool FILT>
+HM_DEV bool layer0(const SearchArgs &a, const Lds &L, Visited &V, Query &Q, int slot,
+                   uint64_t entry_k, uint32_t entry_id, int &rb, int &rn) {
+    const int lane = lane_id();
+    const int ef = Q.ef;
+    uint64_t *const sk = a.spill_key + (size_t)slot * a.spill_cap;
+    uint32_t *const si = a.spill_id + (size_t)slot * a.spill_cap;
+
+    visited_reset(a, L, V, a.h_log2);
+    if (lane == 0) visit(L, V, entry_id);
+    V.count = 1;
+
+    rb = 0;
+    rn = (!FILT || allowed(Q, entry_id)) ? 1 : 0;
+    int cb = 0, cbeg = 0, cend = 1;
+    if (lane == 0) {
+        L.rk[0][0] = entry_k;
+        L.ri[0][0] = entry_id;
+        L.ck[0][0] = entry_k;
+        L.ci[0][0] = entry_id;
+    }
+    wave_sync();
+    uint64_t wk = entry_k;  // worst key of R (meaningful when rn > 0)
+    int spn = 0;
+    uint64_t smk = kInfKey;
+    uint32_t smi = kInfId;
+
+    for (;;) {
+        if (cbeg == cend) {
+            if (spn == 0) break;
+            // Refill C from the spill: prune what can never be expanded, then move the smallest
+            // prefix (at most c_cap) into LDS and sort it.
+            __threadfence();
+            const bool full = rn == ef;
+            int w = 0;
+            for (int b = 0; b < spn; b += 64) {
+                const int j = b + lane;
+                const bool v = j < spn;
+                const uint64_t k = v ? sk[j] : kInfKey;
+                const uint32_t id = v ? si[j] : kInfId;
+                const bool kp = v && (!full || k <= wk);
+                const uint64_t m = ballot(kp);
+                if (kp) {
+                    const int d = w + rank_below(m);
+                    sk[d] = k;
+                    si[d] = id;
+                }
+                w += popc(m);
+            }
+            __threadfence();
+            if (w == 0) break;
+            uint64_t bk = kInfKey;
+            uint32_t bi = kInfId;
+            if (w > a.c_cap) spill_boundary(L, sk, si, w, a.c_cap, bk, bi);
+            int nc = 0, nw = 0;
+            uint64_t mk = kInfKey;
+            uint32_t mi = kInfId;
+            for (int b = 0; b < w; b += 64) {
+                const int j = b + lane;
+                const bool v = j < w;
+                const uint64_t k = v ? sk[j] : kInfKey;
+                const uint32_t id = v ? si[j] : kInfId;
+                const bool sel = v && less(k, id, bk, bi);
+                const bool rest = v && !sel;
+                const uint64_t m1 = ballot(sel), m2 = ballot(rest);
+                if (sel) {
+                    const int d = nc + rank_below(m1);
+                    L.ck[0][d] = k;
+                    L.ci[0][d] = id;
+                }
+                if (rest) {
+                    const int d = nw + rank_below(m2);
+                    sk[d] = k;
+                    si[d] = id;
+                    if (less(k, id, mk, mi)) {
+                        mk = k;
+                        mi = id;
+                    }
+                }
+                nc += popc(m1);
+                nw += popc(m2);
+            }
+            __threadfence();
+            // A refill always moves at least one candidate; anything else would repeat forever.
+            if (nc == 0) return false;
+            wave_min_pair(mk, mi);
+            smk = mk;
+            smi = mi;
+            spn = nw;
+            int p2 = 1;
+            while (p2 < nc) p2 <<= 1;
+            bitonic_sort(L.ck[0], L.ci[0], nc, p2);
+            cb = 0;
+            cbeg = 0;
+            cend = nc;
+        }
+        const uint64_t ck = L.ck[cb][cbeg];
+        const uint32_t cid = L.ci[cb][cbeg];
+        if (FILT) {
+            if (rn == ef && ck > wk) break;
+        } else if (ck > wk) {
+            break;
+        }
+        cbeg += 1;
+        Q.hops += 1;
+
+        for (int base = 0; base < a.w0; base += 64) {
+            const int j = base + lane;
+            const int32_t nb = j < a.w0 ? a.links0[(size_t)cid * a.w0 + j] : -1;
+            visited_make_room(L, V, 64);
+            const bool isnew = nb >= 0 && visit(L, V, (uint32_t)nb);
+            const uint64_t nm = ballot(isnew);
+            const int nnew = popc(nm);
+            if (nnew == 0) continue;
+            if (isnew) L.nbi[rank_below(nm)] = (uint32_t)nb;
+            wave_sync();
+            V.count += nnew;
+            const uint64_t ki = distances<MODE>(a, L, nnew, Q.qq);
+            Q.dcount += nnew;
+            const uint32_t ii = lane < nnew ? L.nbi[lane] : kInfId;
+            const bool al = FILT && lane < nnew ? allowed(Q, ii) : true;
+            L.nbk[lane] = ki;
+            L.nbal[lane] = al ? 1u : 0u;
+            wave_sync();
+
+            // Parallel form of the serial acceptance rule.
+            bool acc = false;
+            if (lane < nnew) {
+                int cnt = upper_count_key(L.rk[rb], rn, ki);
+                for (int q = 0; q < lane; ++q) {
+                    if (L.nbk[q] <= ki && (!FILT || L.nbal[q])) cnt += 1;
+                }
+                acc = cnt < ef;
+            }
+            const uint64_t am = ballot(acc);
+            if (am == 0) continue;
+            const uint64_t rm = FILT ? ballot(acc && al) : am;
+            const int na = popc(am), nr = popc(rm);
+            int ra = 0, rr = 0;
+            if (acc) {
+                for (int q = 0; q < nnew; ++q) {
+                    if (!((am >> q) & 1ull)) continue;
+                    if (less(L.nbk[q], L.nbi[q], ki, ii)) {
+                        ra += 1;
+                        if ((rm >> q) & 1ull) rr += 1;
+                    }
+                }
+                L.sak[ra] = ki;
+                L.sai[ra] = ii;
+                if ((rm >> lane) & 1ull) {
+                    L.srk[rr] = ki;
+                    L.sri[rr] = ii;
+                }
+            }
+            wave_sync();
+
+            if (nr > 0) {  // R <- ef smallest of R and the accepted allowed entries
+                const int nrb = rb ^ 1;
+                const int nrn = min(ef, rn + nr);
+                if ((rm >> lane) & 1ull) {
+                    const int p = lower_count(L.rk[rb], L.ri[rb], rn, ki, ii) + rr;
+                    if (p < ef) {
+                        L.rk[nrb][p] = ki;
+                        L.ri[nrb][p] = ii;
+                    }
+                }
+                for (int q = lane; q < rn; q += 64) {
+                    const uint64_t k = L.rk[rb][q];
+                    const uint32_t id = L.ri[rb][q];
+                    const int p = q + lower_count(L.srk, L.sri, nr, k, id);
+                    if (p < ef) {
+                        L.rk[nrb][p] = k;
+                        L.ri[nrb][p] = id;
+                    }
+                }
+                wave_sync();
+                rb = nrb;
+                rn = nrn;
+                wk = L.rk[rb][rn - 1];
+            }
+
+            {  // C <- live candidates and all accepted entries, pruned, overflow spilled
+                const int live = cend - cbeg;
+                const uint64_t *ck0 = L.ck[cb] + cbeg;
+                const uint32_t *ci0 = L.ci[cb] + cbeg;
+                const bool full = rn == ef;
+                const int total = live + na;
+                const int cut_p =
+                    full ? upper_count_key(ck0, live, wk) + upper_count_key(L.sak, na, wk) : total;
+                const int cut_s = spn > 0 ? lower_count(ck0, ci0, live, smk, smi) +
+                                                lower_count(L.sak, L.sai, na, smk, smi)
+                                          : total;
+                const int keep = min(a.c_cap, min(cut_p, cut_s));
+                const int nsp = cut_p - keep;
+                if (nsp > 0 && (int64_t)spn + nsp > a.spill_cap) return false;
+                const int ncb = cb ^ 1;
+                if (acc) {
+                    const int p = lower_count(ck0, ci0, live, ki, ii) + ra;
+                    if (p < keep) {
+                        L.ck[ncb][p] = ki;
+                        L.ci[ncb][p] = ii;
+                    } else if (p < cut_p) {
+                        sk[spn + p - keep] = ki;
+                        si[spn + p - keep] = ii;
+                        if (p == keep) {
+                            L.misc[0] = ki;
+                            L.misc[1] = ii;
+                        }
+                    }
+                }
+                for (int q = lane; q < live; q += 64) {
+                    const uint64_t k = ck0[q];
+                    const uint32_t id = ci0[q];
+                    const int p = q + lower_count(L.sak, L.sai, na, k, id);
+                    if (p < keep) {
+                        L.ck[ncb][p] = k;
+                        L.ci[ncb][p] = id;
+                    } else if (p < cut_p) {
+                        sk[spn + p - keep] = k;
+                        si[spn + p - keep] = id;
+                        if (p == keep) {
+                            L.misc[0] = k;
+                            L.misc[1] = id;
+                        }
+                    }
+                }
+                wave_sync();
+                if (nsp > 0) {
+                    const uint64_t fk = L.misc[0];
+                    const uint32_t fi = (uint32_t)L.misc[1];
+                    if (less(fk, fi, smk, smi)) {
+                        smk = fk;
+                        smi = fi;
+                    }
+                    spn += nsp;
+                }
+                cb = ncb;
+                cbeg = 0;
+                cend = keep;
+            }
+        }
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------- one query
+
+template <int MODE, bool FILT>
+HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int slot) {
+    const int lane = lane_id();
+    Query Q;
+    Q.ef = a.ef_q ? a.ef_q[qi] : a.ef_all;
+    Q.k = a.k_q ? a.k_q[qi] : a.k_all;
+    Q.mask = nullptr;
+    if (FILT) {
+        const int mi = a.mask_idx ? a.mask_idx[qi] : a.mask_all;
+        if (mi >= 0) Q.mask = a.mask_pool + (size_t)mi * a.mask_words;
+    }
+    Q.dcount = 0;
+    Q.hops = 0;
+    Q.qq = 0;
+
+    // Query into LDS.
+    if (MODE == kExactU8) {
+        const float *q = a.q_f32 + (size_t)qi * a.dim;
+        int32_t s = 0;
+        for (int i = lane; i < a.dimp; i += 64) {
+            const int v = i < a.dim ? (int)q[i] : 0;
+            L.qb[i] = (uint8_t)v;
+            s += v * v;
+        }
+        Q.qq = wave_sum(s);
+    } else {
+        double *qd = (double *)L.qb;
+        for (int i = lane; i < a.dim; i += 64)
+            qd[i] = MODE == kSeqU8 ? a.q_f64[(size_t)qi * a.dim + i]
+                                   : (double)a.q_f32[(size_t)qi * a.dim + i];
+    }
+    if (lane == 0) L.nbi[0] = (uint32_t)a.entry;
+    wave_sync();
+
+    // Entry point distance (numba_engine.py:449-451), then greedy descent.
+    uint64_t cur_k = shfl_u64(distances<MODE>(a, L, 1, Q.qq), 0);
+    uint32_t cur_id = (uint32_t)a.entry;
+    Q.dcount = 1;
+    wave_sync();
+    for (int layer = a.max_level; layer > 0; --layer)
+        greedy_layer<MODE>(a, L, V, Q, layer, cur_k, cur_id);
+
+    int rb = 0, rn = 0;
+    const bool ok = layer0<MODE, FILT>(a, L, V, Q, slot, cur_k, cur_id, rb, rn);
+    if (!ok) rn = 0;
+
+    int64_t *oi = a.out_ids + (size_t)qi * a.kstride;
+    double *od = a.out_d + (size_t)qi * a.kstride;
+    const int take = min(Q.k, rn);
+    for (int i = lane; i < a.kstride; i += 64) {
+        if (i < take) {
+            oi[i] = (int64_t)L.ri[rb][i];
+            od[i] = decode<MODE>(L.rk[rb][i], a.metric);
+        } else {
+            oi[i] = -1;
+            od[i] = __builtin_huge_val();
+        }
+    }
+    if (lane == 0) {
+        a.out_dc[qi] = Q.dcount;
+        a.out_hops[qi] = Q.hops;
+        a.out_fail[qi] = ok ? 0 : 1;
+    }
+    wave_sync();
+}
+
+template <int MODE, bool FILT>
+__global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
+    const int slot = blockIdx.x;
+    const Lds L = carve(lds_raw, a, MODE);
+    Visited V;
+    V.bits = a.bitsets + (size_t)slot * a.bitset_words;
+    V.dirty = false;
+    V.use_bits = false;
+    V.hlog = a.h_log2;
+    V.count = 0;
+    for (;;) {
+        int qi = 0;
+        if (lane_id() == 0) qi = atomicAdd(a.queue, 1);
+        qi = uniform(shfl(qi, 0));
+        if (qi >= a.nq) break;
+        run_query<MODE, FILT>(a, L, V, qi, slot);
+    }
+    // Leave the bitset clean for the next launch.
+    if (V.dirty) {
+        for (int64_t w = lane_id(); w < a.bitset_words; w += 64) V.bits[w] = 0u;
+    }
+}
+
+__global__ void reset_queue_kernel(int32_t *q) { q[0] = 0; }
+
+template __global__ void search_kernel<kExactU8, false>(SearchArgs);
+template __global__ void search_kernel<kExactU8, true>(SearchArgs);
+template __global__ void search_kernel<kSeqF32, false>(SearchArgs);
+template __global__ void search_kernel<kSeqF32, true>(SearchArgs);
+template __global__ void search_kernel<kSeqU8, false>(SearchArgs);
+
+}  // namespace hm
+
+// ---------------------------------------------------------------------------- host launch
+
+namespace hm {
+
+using KernelFn = void (*)(SearchArgs);
+
+static KernelFn pick(int mode, bool filt) {
+    if (mode == kExactU8) return filt ? search_kernel<kExactU8, true> : search_kernel<kExactU8, false>;
+    if (mode == kSeqF32) return filt ? search_kernel<kSeqF32, true> : search_kernel<kSeqF32, false>;
+    return search_kernel<kSeqU8, false>;
+}
+
+// Blocks (one wave each) that fit on the device at once for this LDS size.
+int search_slots(int mode, bool filt, int lds, int device_cus) {
+    KernelFn fn = pick(mode, filt);
+    if (lds > 64 * 1024)
+        hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, 64, lds) !=
+            hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    return per_cu * device_cus;
+}
+
+hipError_t search_launch(int mode, bool filt, const SearchArgs &a, int blocks, hipStream_t s) {
+    KernelFn fn = pick(mode, filt);
+    if (a.lds_per_wave > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)fn,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           a.lds_per_wave);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, s, a.queue);
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(64), a.lds_per_wave, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace hm

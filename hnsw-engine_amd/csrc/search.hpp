@@ -1,0 +1,75 @@
+// Shared definitions between the search kernels and the host launch code.
+#pragma once
+#include <stdint.h>
+
+namespace hm {
+
+enum Metric : int { kL2 = 0, kIP = 1, kCosine = 2 };
+
+// Distance kernels the search can run with (chosen per batch on the host):
+//   kExactU8  vectors and queries are 8-bit integers 0..255; distances are exact integers
+//             computed with packed byte dot products (any order is exact, so the result equals
+//             the reference's float64 sum bit for bit);
+//   kSeqF32   float32 vectors, float64 query, float64 sequential unfused sum (the reference's
+//             numba_engine.py:57-70 order);
+//   kSeqU8    uint8 codes, float64 code-space query, the same sequential sum
+//             (numba_engine.py:674-701).
+enum DistMode : int { kExactU8 = 0, kSeqF32 = 1, kSeqU8 = 2 };
+
+struct SearchArgs {
+    // graph, repacked: rows of w0 / w1 int32 with -1 beyond the used count
+    const int32_t *links0;
+    const int32_t *links_up;
+    const int32_t *upper_off;  // (n,) row of layer 1 in links_up, -1 for level-0 nodes
+    int64_t n;
+    int w0, w1;
+    int entry, max_level;
+    // vectors
+    const float *vec_f32;   // (n, dim)
+    const uint8_t *vec_u8;  // (n, dimp): exact 8-bit copy (kExactU8) or codes (kSeqU8)
+    int dim, dimp;
+    int metric;
+    // queries: float32 (kExactU8, kSeqF32) or float64 (kSeqU8), (nq, dim)
+    const float *q_f32;
+    const double *q_f64;
+    int nq;
+    const int32_t *ef_q;  // per query, already max(ef, k); null: ef_all
+    const int32_t *k_q;   // per query; null: k_all
+    int ef_all, k_all;
+    int kstride;
+    // filters: a pool of bitsets (mask_words uint32 each); per-query index or one for all
+    const uint32_t *mask_pool;
+    int64_t mask_words;
+    const int32_t *mask_idx;  // per query (-1: unfiltered); null: mask_all
+    int mask_all;             // -1: unfiltered
+    // outputs
+    int64_t *out_ids;
+    double *out_d;
+    int32_t *out_dc;
+    int32_t *out_hops;
+    int32_t *out_fail;  // 1: candidate spill overflowed, rerun with a larger spill
+    // persistent work queue and per-wave scratch (slot = global wave index)
+    int32_t *queue;
+    uint32_t *bitsets;
+    int64_t bitset_words;
+    uint64_t *spill_key;
+    uint32_t *spill_id;
+    int64_t spill_cap;
+    int r_cap, c_cap, h_log2;
+    int lds_per_wave;
+    int slots;
+};
+
+// Bytes of LDS one wave needs for the given capacities.
+inline int lds_bytes_per_wave(int mode, int dim, int dimp, int r_cap, int c_cap, int h_log2) {
+    auto up = [](int v) { return (v + 15) & ~15; };
+    int q = mode == kExactU8 ? up(dimp) : up(dim * 8);
+    int r = 2 * (up(r_cap * 8) + up(r_cap * 4));
+    int c = 2 * (up(c_cap * 8) + up(c_cap * 4));
+    int stage = 2 * (64 * 8 + 64 * 4) + 64 * 8 + 64 * 4 + 64 * 4 + 64 * 4;
+    int hash = 4 << h_log2;
+    int misc = 256 * 4 + 64;
+    return q + r + c + stage + hash + misc;
+}
+
+}  // namespace hm

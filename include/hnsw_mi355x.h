@@ -1,0 +1,150 @@
+/*
+ * C ABI of the MI355X search engine for hnsw-engine graphs (libhnsw_mi355x.so).
+ *
+ * Every entry point takes plain pointers and sizes and returns 0 on success or a negative
+ * status; hm_last_error() then holds the message (the Python layer re-raises it as the
+ * reference's exception type). Host-pointer entry points copy to and from HBM themselves;
+ * the *_dev variants take device pointers and a HIP stream and allocate nothing per call
+ * beyond the scratch the handle keeps between calls.
+ *
+ * Reference interfaces replaced (paths relative to the hnsw-engine repository):
+ *   hm_open            NumbaEngine.attach_native      src/hnsw_engine/numba_engine.py:827-868
+ *   hm_search          NumbaEngine.batch_search       src/hnsw_engine/numba_engine.py:652-672
+ *                      (-> _search_batch :422-503, _search_layer :98-148,
+ *                          _search_layer0_filtered :151-198)
+ *   hm_attach_codes    the codes QuantizedIndex keeps src/hnsw_engine/quantization.py:95-102
+ *   hm_search_codes    NumbaEngine.batch_search_codes src/hnsw_engine/numba_engine.py:674-701
+ *   hm_rerank          NumbaEngine.rerank             src/hnsw_engine/numba_engine.py:703-713
+ *                      (-> _rerank_batch :506-530)
+ *   hm_exact           exact_search                   src/hnsw_engine/oracle.py:35-126
+ *                      (masked: exact_subset_search   benchmarks/run_filtered.py:76-106)
+ *   hm_build           NumbaEngine.insert             src/hnsw_engine/numba_engine.py:609-643
+ *                      (-> _insert_batch :287-419; host C++, not a GPU kernel)
+ */
+#ifndef HNSW_MI355X_H
+#define HNSW_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { HM_L2 = 0, HM_IP = 1, HM_COSINE = 2 };
+
+enum {
+    HM_OK = 0,
+    HM_EINVAL = -1,  /* bad argument (shape, dtype, range) */
+    HM_EDEVICE = -2, /* HIP error */
+    HM_ENOMEM = -3
+};
+
+/* The reference's native arrays (numba_engine.py:814-825), row-major, as save_native writes
+ * them: links0 is (n, 2M+1), links_up is (rows, M+1); the first count slots of a row are used. */
+typedef struct {
+    int64_t n;
+    int32_t dim;
+    int32_t M;
+    int32_t metric;
+    int32_t max_level;
+    int64_t entry_point;
+    int64_t rows;
+    const float *vectors;        /* (n, dim) */
+    const int64_t *levels;       /* (n,) */
+    const int32_t *links0;       /* (n, 2M+1) */
+    const int32_t *count0;       /* (n,) */
+    const int64_t *upper_offset; /* (n,) */
+    const int32_t *links_up;     /* (rows, M+1) */
+    const int32_t *count_up;     /* (rows,) */
+} hm_native_graph;
+
+typedef struct hm_index hm_index;
+
+/* Per-batch search options. k and ef are per query (arrays of nq) or, when the array is null,
+ * k_all / ef_all for every query; the engine searches layer 0 with max(ef, k) as the reference
+ * does (numba_engine.py:473,487). Masks: a pool of n_masks boolean rows of n bytes
+ * (mask_pool, n_masks x n); mask_index gives each query its row (-1: unfiltered) or, when
+ * null, mask_all applies to every query (-1: unfiltered). Outputs are (nq, kstride) with
+ * -1 / +inf padding; kstride >= max k. */
+typedef struct {
+    int64_t nq;
+    const int32_t *k;
+    const int32_t *ef;
+    int32_t k_all;
+    int32_t ef_all;
+    int32_t kstride;
+    const uint8_t *mask_pool;
+    int64_t n_masks;
+    const int32_t *mask_index;
+    int32_t mask_all;
+} hm_search_opts;
+
+/* Uploads a validated graph to HBM (validation is the caller's: the kernels do not
+ * bounds-check, as the reference's do not; numba_engine.py:878-927). Rows of links are
+ * repacked with -1 beyond their count and repeated links dropped (a repeat is always skipped
+ * as visited, so results do not change). When every vector value is an integer in 0..255 an
+ * exact uint8 copy is kept as well. */
+int hm_open(const hm_native_graph *graph, int device, hm_index **out);
+void hm_close(hm_index *index);
+
+/* 1 if the index keeps the exact 8-bit copy (integer-valued 0..255 vectors, metric l2/ip). */
+int hm_has_exact_u8(const hm_index *index);
+
+/* Algorithm 5 for every query (host buffers). queries: (nq, dim) float32, prepared as the
+ * facade prepares them. ids: int64 (nq, kstride); dists: float64 (nq, kstride); dcount and
+ * hops: int64 (nq,) per-query distance computations and expansions, counted as the reference
+ * counts them (numba_engine.py:130,139,451), or null. */
+int hm_search(hm_index *index, const float *queries, const hm_search_opts *opts, int64_t *ids,
+              double *dists, int64_t *dcount, int64_t *hops);
+
+/* The same search with every buffer already in HBM (queries float32 (nq, dim); outputs
+ * ids int64, dists float64 (nq, kstride), dcount/hops int32 (nq,) or null). Masks are device
+ * bitsets: mask_bits holds n_masks rows of ceil(n/32) uint32 words; mask_index (device, nq) or
+ * mask_all as in hm_search_opts; k/ef pointers in opts must be device pointers when set.
+ * exact_u8 = 1 selects the exact 8-bit kernel: the caller guarantees that every query value
+ * is an integer in 0..255 (hm_queries_exact checks a host batch) and the index has the copy.
+ * d_fail (device, nq int32, or null) is set to 1 for a query whose candidate overflow exceeded
+ * the per-wave spill (its row is then padding; run it again through hm_search, which retries
+ * with room for every node). stream: a hipStream_t (null: the default stream). */
+int hm_search_dev(hm_index *index, const float *d_queries, const hm_search_opts *opts,
+                  const uint32_t *d_mask_bits, int exact_u8, int64_t *d_ids, double *d_dists,
+                  int32_t *d_dcount, int32_t *d_hops, int32_t *d_fail, void *stream);
+
+/* 1 if every value of the host batch is an integer in 0..255 (exact 8-bit kernel eligible). */
+int hm_queries_exact(const float *queries, int64_t count);
+
+/* uint8 codes (n, dim) of a ScalarQuantizer, searched by hm_search_codes. */
+int hm_attach_codes(hm_index *index, const uint8_t *codes);
+
+/* Algorithm 5 over the codes with float64 code-space queries (nq, dim), unfiltered. */
+int hm_search_codes(hm_index *index, const double *queries, const hm_search_opts *opts,
+                    int64_t *ids, double *dists, int64_t *dcount, int64_t *hops);
+
+/* Exact float64 distances of each row's candidate ids (-1 skipped) to the float32 vectors,
+ * best k by (distance, id); -1 / +inf padding. candidates: int64 (nq, fetch). */
+int hm_rerank(hm_index *index, const float *queries, int64_t nq, const int64_t *candidates,
+              int32_t fetch, int32_t k, int64_t *ids, double *dists);
+
+/* Exact k nearest neighbours by (distance, id) of each query among the rows of data (float32
+ * (n, dim)), optionally restricted to allowed rows (bool bytes (n,), or null). The float64
+ * distances are the reference's; the shortlist runs on the matrix cores. */
+int hm_exact(const float *data, int64_t n, int32_t dim, int32_t metric, const float *queries,
+             int64_t nq, int32_t k, const uint8_t *allowed, int device, int64_t *ids,
+             double *dists);
+
+/* Sequential insertion (host C++, the reference's _insert_batch): inserts vectors[start..n)
+ * with their levels into the native arrays, which hold capacity for all n nodes and
+ * sum(levels) upper rows; state = {entry_point, max_level} in and out. selection: 0 simple,
+ * 1 heuristic. upper_offset must already hold each node's first upper row (-1 for level 0). */
+int hm_build(int64_t start, int64_t n, int32_t dim, int32_t metric, int32_t M,
+             int32_t ef_construction, int32_t selection, int32_t extend_candidates,
+             int32_t keep_pruned, const float *vectors, const int64_t *levels, int32_t *links0,
+             int32_t *count0, const int64_t *upper_offset, int32_t *links_up, int32_t *count_up,
+             int64_t *state, int64_t *dcount);
+
+const char *hm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
