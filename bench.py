@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Benchmark: QPS at tie-aware recall@10 >= 0.95 on 1M x 128 (one MI355X).
+
+Workload (``hnsw_amd.workloads.clustered_1m``): 1,000,000 x 128 float32 vectors with integer
+values 0..255 from 1,000 Gaussian clusters, 10,000 queries, L2, M = 16, ef_construction = 200;
+the graph is built by sequential insertion on the host (cached under ``.graph_cache/``).
+
+Steps: exact top-10 ground truth with ``exact_search`` (GPU); a sweep of ef over
+{16, 32, 64, 128, 256, 512} picks the smallest ef whose recall@10 >= 0.95; then W untimed and
+K timed steps, a step being one batch search of all 10,000 queries with queries and outputs
+already in HBM. The JSON line carries the roofline of the search kernel (algorithmic bytes per
+launch over the launch's average duration, HIP events on the launch stream) and the CPU
+baseline (the C oracle, 1 thread and all cores, on a bounded query sample, ids checked equal).
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path[:0] = [str(ROOT / "hnsw-engine_amd"), str(ROOT)]
+
+HBM_PEAK_GBS = 8000.0
+INT8_PEAK_TOPS = 5000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def exact_l2(data: np.ndarray, queries: np.ndarray, ids: np.ndarray) -> np.ndarray:
+    """Exact squared distances of the returned ids (integer data: exact in int64)."""
+    out = np.full(ids.shape, np.inf)
+    valid = ids >= 0
+    rows = data[np.where(valid, ids, 0)].astype(np.int64)
+    diff = rows - queries.astype(np.int64)[:, None, :]
+    d = (diff * diff).sum(axis=2).astype(np.float64)
+    out[valid] = d[valid]
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="clustered_1m")
+    ap.add_argument("--k", type=int, default=10)
+    ap.add_argument("--target-recall", type=float, default=0.95)
+    ap.add_argument("--sweep", default="16,32,64,128,256,512")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0,
+                    help="target CPU time of each CPU-baseline leg")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            f"bench.py measures one GPU only (asked for {args.gpus}); the search path shards by "
+            "replicating the index and splitting queries, which needs no collective.",
+            file=sys.stderr,
+        )
+        return 2
+
+    import torch
+
+    from hnsw_amd import _native
+    from hnsw_amd.exact import exact_search
+    from hnsw_amd.recall import recall_at_k
+    from hnsw_amd.workloads import WORKLOADS, load_or_build
+
+    w = WORKLOADS[args.workload]
+    eng, data, queries = load_or_build(w, log=log)
+    nq, k = queries.shape[0], args.k
+    handle = eng._device()
+    lib = _native.lib()
+
+    log("[bench] exact ground truth on the GPU")
+    t0 = time.perf_counter()
+    true_ids, true_d = exact_search(data, queries, k)
+    t_exact = time.perf_counter() - t0
+
+    exact_u8 = bool(lib.hm_has_exact_u8(handle)) and bool(
+        lib.hm_queries_exact(_native.ptr(queries), queries.size))
+    dev = torch.device("cuda:0")
+    q_dev = torch.from_numpy(queries).to(dev)
+    ids_dev = torch.empty((nq, k), dtype=torch.int64, device=dev)
+    d_dev = torch.empty((nq, k), dtype=torch.float64, device=dev)
+    dc_dev = torch.empty(nq, dtype=torch.int32, device=dev)
+    hops_dev = torch.empty(nq, dtype=torch.int32, device=dev)
+    fail_dev = torch.zeros(nq, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def launch(ef: int) -> None:
+        opts = _native.SearchOpts()
+        opts.nq, opts.k_all, opts.ef_all, opts.kstride, opts.mask_all = nq, k, ef, k, -1
+        _native.check(lib.hm_search_dev(
+            handle, ctypes.c_void_p(q_dev.data_ptr()), ctypes.byref(opts), None, int(exact_u8),
+            ctypes.c_void_p(ids_dev.data_ptr()), ctypes.c_void_p(d_dev.data_ptr()),
+            ctypes.c_void_p(dc_dev.data_ptr()), ctypes.c_void_p(hops_dev.data_ptr()),
+            ctypes.c_void_p(fail_dev.data_ptr()), ctypes.c_void_p(stream.cuda_stream)))
+
+    sweep = []
+    for ef in [int(v) for v in args.sweep.split(",")]:
+        launch(ef)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        launch(ef)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        if int(fail_dev.sum()) != 0:
+            raise RuntimeError("a query overflowed its candidate spill")
+        ids = ids_dev.cpu().numpy()
+        rec = recall_at_k(ids, exact_l2(data, queries, ids), true_ids, true_d, k)
+        dc = dc_dev.cpu().numpy().astype(np.int64)
+        hops = hops_dev.cpu().numpy().astype(np.int64)
+        sweep.append({"ef": ef, "recall": rec["recall"], "recall_ids": rec["recall_ids"],
+                      "qps": nq / dt, "dcount_mean": float(dc.mean()),
+                      "hops_mean": float(hops.mean()), "ids": ids, "dc": dc, "hops": hops})
+        log(f"[bench] ef={ef} recall={rec['recall']:.4f} qps={nq / dt:,.0f} "
+            f"dist/q={dc.mean():.0f} hops/q={hops.mean():.0f}")
+    chosen = next((s for s in sweep if s["recall"] >= args.target_recall), sweep[-1])
+    ef = chosen["ef"]
+
+    for _ in range(args.warmup):
+        launch(ef)
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    t_begin = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    t_begin.record(stream)
+    for i in range(args.steps):
+        starts[i].record(stream)
+        launch(ef)
+        ends[i].record(stream)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t_begin.elapsed_time(t_end)
+    launch_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    qps = nq * args.steps / (total_ms / 1e3)
+    ids_final = ids_dev.cpu().numpy()
+    assert np.array_equal(ids_final, chosen["ids"]), "non-deterministic search results"
+
+    # Algorithmic bytes of one launch: every distance reads one stored row, every expansion one
+    # links row (upper-layer rows counted at the layer-0 width), as the reference counts them.
+    row_bytes = eng.dim if exact_u8 else eng.dim * 4
+    if exact_u8:
+        row_bytes = (eng.dim + 15) // 16 * 16
+    link_bytes = ((2 * eng.M + 15) // 16 * 16) * 4
+    algo_bytes = float(chosen["dc"].sum() * row_bytes + chosen["hops"].sum() * link_bytes)
+    achieved = algo_bytes / (launch_ms / 1e3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / f"search_{args.workload}_traffic.json"
+    if tfile.is_file():
+        tj = json.loads(tfile.read_text())
+        if tj.get("ef") == ef:
+            traffic = tj.get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(eng, queries, k, ef, chosen["ids"], args.cpu_seconds)
+        cpu["ids_match_gpu"] = bool(cpu.pop("match"))
+
+    line = {
+        "metric": "QPS at recall@10>=0.95, 1M x 128 fp32, learned-stop vs plain beam, 1 GPU",
+        "value": qps,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8-exact" if exact_u8 else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: {w.n} x {w.dim} float32, integer values 0..255, "
+            f"{w.n_clusters} clusters, {nq} queries, L2, M={w.M}, efC={w.ef_construction}",
+            "k": k, "ef": ef, "recall": chosen["recall"], "variant": "plain beam",
+            "sweep": [{kk: s[kk] for kk in ("ef", "recall", "qps", "dcount_mean", "hops_mean")}
+                      for s in sweep],
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": "search_kernel", "launch_ms": launch_ms, "algo_bytes": algo_bytes,
+        },
+        "oracle": {"seconds_incl_upload": t_exact,
+                   "tops_incl_upload": 2.0 * nq * w.n * w.dim / t_exact / 1e12},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_baseline(eng, queries, k: int, ef: int, gpu_ids: np.ndarray, seconds: float) -> dict:
+    """The C oracle on the host: 1 thread and all cores, bounded samples, ids checked."""
+    import oracle
+
+    arrays = eng.native_arrays()
+    cores = oracle.threads_available()
+    log(f"[bench] CPU baseline (C oracle), {cores} cores")
+    probe = 100
+    t = time.perf_counter()
+    ids, _, _, _ = oracle.search_batch(arrays, eng.entry_point, eng.max_level, eng.metric,
+                                       queries[:probe], k, ef, threads=1)
+    per_q = (time.perf_counter() - t) / probe
+    match = bool(np.array_equal(ids, gpu_ids[:probe]))
+    n1 = int(min(queries.shape[0], max(probe, seconds / per_q)))
+    t = time.perf_counter()
+    ids1, _, _, _ = oracle.search_batch(arrays, eng.entry_point, eng.max_level, eng.metric,
+                                        queries[:n1], k, ef, threads=1)
+    qps1 = n1 / (time.perf_counter() - t)
+    match &= bool(np.array_equal(ids1, gpu_ids[:n1]))
+    nall = queries.shape[0]
+    reps = max(1, int(seconds / (nall / (qps1 * cores))))
+    t = time.perf_counter()
+    for _ in range(reps):
+        idsa, _, _, _ = oracle.search_batch(arrays, eng.entry_point, eng.max_level, eng.metric,
+                                            queries, k, ef, threads=cores)
+    qpsa = nall * reps / (time.perf_counter() - t)
+    match &= bool(np.array_equal(idsa, gpu_ids))
+    log(f"[bench] CPU 1-thread {qps1:,.0f} q/s ({n1} queries); {cores} threads {qpsa:,.0f} q/s")
+    return {
+        "value": qpsa, "unit": "queries/s", "cores": cores, "kind": "port",
+        "sample": f"all {nall} queries x {reps} at ef={ef} on {cores} threads; "
+                  f"1 thread: {qps1:.0f} q/s on the first {n1} queries",
+        "single_thread_qps": qps1, "match": match,
+    }
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,15 +1,522 @@
-// Exact k-nearest-neighbour search (the reference's oracle.py:35-126) on the GPU.
+// Exact k-nearest-neighbour search on the GPU (the reference's oracle, oracle.py:35-126, and the
+// masked brute force of benchmarks/run_filtered.py:76-106): the k smallest (distance, id)
+// pairs of each query over all (allowed) rows, distances in float64, ties to the lower id.
+//
+// Two distance producers feed one top-k consumer:
+//   * 8-bit integer data and queries (values 0..255, l2 or ip): int8 matrix cores. Values are
+//     shifted to -128..127; v_mfma_i32_32x32x32_i8 accumulates x'.q' exactly in int32 and
+//     |x-q|^2 = |x'|^2 + |q'|^2 - 2 x'.q' (ip: x.q = x'.q' + 128 (sum x' + sum q') + 128^2 dim)
+//     is an exact integer, so no error margin and no re-rank are needed: the float64 value of
+//     that integer is the reference's float64 distance bit for bit.
+//   * anything else: float64 on the vector units, each (query, row) pair summed left to right
+//     without fusion; cosine divides rows and queries by their float64 norms first
+//     (oracle.py:228-229).
+// Consumer: every work-group owns a block of queries and a chunk of rows and keeps, per query, a
+// small LDS buffer of candidates below a threshold (the k-th smallest (key, id) seen so far);
+// when a buffer nears capacity it is sorted and cut to k. Per-chunk top-k lists are merged per
+// query by a second kernel.
 #include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
 
 #include "../../include/hnsw_mi355x.h"
+#include "common.hpp"
 #include "index.hpp"
+#include "keys.hpp"
+#include "wave.hpp"
+
+typedef __attribute__((ext_vector_type(4))) int i32x4;
+typedef __attribute__((ext_vector_type(16))) int i32x16;
+
+namespace hm {
+
+struct ExactArgs {
+    int64_t n;
+    int dim, dimp, metric, k, nq;
+    // float64 producer
+    const float *x32, *q32;
+    const double *xn, *qn;  // cosine: rows / queries divided by their norms
+    // int8 producer: (rows, dimp) shifted bytes, zero padded; per-row/query sums
+    const int8_t *x8, *q8;
+    const int32_t *xsq, *qsq, *xsum, *qsum;
+    const uint32_t *allowed;  // bitset or null
+    int64_t chunk;
+    int nchunks;
+    int cb;  // per-query buffer capacity (power of two)
+    uint64_t *part_key;
+    uint32_t *part_id;
+};
+
+// ---------------------------------------------------------------------------- top-k buffers
+
+struct TopK {
+    uint64_t *key;  // [qb * cb]
+    uint32_t *id;
+    int *cnt;       // [qb]
+    uint64_t *tk;   // [qb] threshold
+    uint32_t *ti;
+    int cb, k;
+};
+
+HM_DEV void topk_insert(const TopK &T, int q, uint64_t key, uint32_t id) {
+    if (!less(key, id, T.tk[q], T.ti[q])) return;
+    const int pos = atomicAdd(&T.cnt[q], 1);
+    T.key[q * T.cb + pos] = key;
+    T.id[q * T.cb + pos] = id;
+}
+
+// Sorts query q's buffer and keeps its k smallest; the k-th becomes the threshold. One wave.
+HM_DEV void topk_compact(const TopK &T, int q) {
+    const int m = min(T.cnt[q], T.cb);
+    uint64_t *k = T.key + q * T.cb;
+    uint32_t *id = T.id + q * T.cb;
+    bitonic_sort(k, id, m, T.cb);
+    const int keep = min(m, T.k);
+    if (lane_id() == 0) {
+        T.cnt[q] = keep;
+        if (keep == T.k) {
+            T.tk[q] = k[T.k - 1];
+            T.ti[q] = id[T.k - 1];
+        }
+    }
+    wave_sync();
+}
+
+HM_DEV TopK topk_carve(uint8_t *lds, int qb, int cb, int k) {
+    TopK T;
+    T.key = (uint64_t *)lds;
+    T.id = (uint32_t *)(lds + (size_t)qb * cb * 8);
+    T.cnt = (int *)(lds + (size_t)qb * cb * 12);
+    T.tk = (uint64_t *)(lds + (size_t)qb * cb * 12 + qb * 4 + ((qb * 4) % 8));
+    T.ti = (uint32_t *)((uint8_t *)T.tk + qb * 8);
+    T.cb = cb;
+    T.k = k;
+    return T;
+}
+
+HM_DEV int topk_bytes(int qb, int cb) { return qb * cb * 12 + qb * 4 + ((qb * 4) % 8) + qb * 12; }
+
+HM_DEV bool row_allowed(const ExactArgs &a, int64_t j) {
+    return a.allowed == nullptr || ((a.allowed[j >> 5] >> (j & 31)) & 1u);
+}
+
+// Writes each query's k smallest of this chunk to the partial lists.
+HM_DEV void topk_flush(const ExactArgs &a, const TopK &T, int q0, int qcount, int wave,
+                       int nwaves, int chunk) {
+    for (int q = wave; q < qcount; q += nwaves) {
+        topk_compact(T, q);
+        const int m = T.cnt[q];
+        const size_t base = ((size_t)(q0 + q) * a.nchunks + chunk) * a.k;
+        for (int t = lane_id(); t < a.k; t += 64) {
+            a.part_key[base + t] = t < m ? T.key[q * T.cb + t] : kInfKey;
+            a.part_id[base + t] = t < m ? T.id[q * T.cb + t] : kInfId;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------- int8 producer
+
+// Work-group: 1 or 2 waves (blockDim / 64), 32 queries each (one 32x32 MFMA row block), all
+// rows of one chunk in blocks of 32. Operands: lane l holds row/column l % 32 and bytes [16 (l / 32), +16) of each
+// 32-byte slice of the dimension, identically for A and B, so the products pair up.
+template <int MAXCH>
+__global__ __launch_bounds__(128) void exact_i8_kernel(ExactArgs a) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+    const int lane = lane_id(), wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const int qb = 32 * nwaves;
+    const int q0 = blockIdx.x * qb;
+    const int chunk = blockIdx.y;
+    const int qcount = min(qb, a.nq - q0);
+    TopK T = topk_carve(lds, qb, a.cb, a.k);
+    for (int q = threadIdx.x; q < qb; q += blockDim.x) {
+        T.cnt[q] = 0;
+        T.tk[q] = kInfKey;
+        T.ti[q] = kInfId;
+    }
+    __syncthreads();
+    const int nch = a.dimp >> 5;
+    const int half = lane >> 5, col = lane & 31;
+    // A operand (queries) stays in registers for the whole chunk.
+    const int my_q = q0 + wave * 32 + col;
+    i32x4 av[MAXCH];
+#pragma unroll
+    for (int c = 0; c < MAXCH; ++c) {
+        av[c] = i32x4{0, 0, 0, 0};
+        if (c < nch && my_q < a.nq)
+            av[c] = *(const i32x4 *)(a.q8 + (size_t)my_q * a.dimp + c * 32 + half * 16);
+    }
+    // Query terms for the 16 accumulator rows this lane sees: i = (r%4) + 4 half + 8 (r/4).
+    int qterm[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int qi = q0 + wave * 32 + (r & 3) + 4 * half + 8 * (r >> 2);
+        qterm[r] = qi < a.nq ? (a.metric == kL2 ? a.qsq[qi] : a.qsum[qi]) : 0;
+    }
+    const int64_t r_begin = (int64_t)chunk * a.chunk;
+    const int64_t r_end = min(a.n, r_begin + a.chunk);
+    uint64_t tk[16];
+    uint32_t ti[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        tk[r] = kInfKey;
+        ti[r] = kInfId;
+    }
+    for (int64_t r0 = r_begin; r0 < r_end; r0 += 32) {
+        // Keep every buffer able to absorb one block (at most 32 inserts per query).
+        bool need = false;
+        for (int q = wave * 32; q < wave * 32 + 32; ++q) need |= T.cnt[q] > T.cb - 32;
+        if (need) {
+            for (int q = wave * 32; q < wave * 32 + 32; ++q)
+                if (T.cnt[q] > T.cb - 32) topk_compact(T, q);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int ql = wave * 32 + (r & 3) + 4 * half + 8 * (r >> 2);
+                tk[r] = T.tk[ql];
+                ti[r] = T.ti[ql];
+            }
+        }
+        const int64_t j = r0 + col;
+        const bool jv = j < r_end;
+        i32x4 bv[MAXCH];
+#pragma unroll
+        for (int c = 0; c < MAXCH; ++c) {
+            bv[c] = i32x4{0, 0, 0, 0};
+            if (c < nch && jv) bv[c] = *(const i32x4 *)(a.x8 + (size_t)j * a.dimp + c * 32 + half * 16);
+        }
+        i32x16 acc = {0};
+#pragma unroll
+        for (int c = 0; c < MAXCH; ++c)
+            if (c < nch) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[c], bv[c], acc, 0, 0, 0);
+        if (jv && row_allowed(a, j)) {
+            const int32_t xt = a.metric == kL2 ? a.xsq[j] : a.xsum[j];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int ql = wave * 32 + (r & 3) + 4 * half + 8 * (r >> 2);
+                int32_t d;
+                if (a.metric == kL2) d = xt + qterm[r] - 2 * acc[r];
+                else d = -(acc[r] + 128 * (xt + qterm[r]) + 16384 * a.dim);
+                const uint64_t key = key_of_exact(d, a.metric);
+                if (q0 + ql < a.nq && less(key, (uint32_t)j, tk[r], ti[r]))
+                    topk_insert(T, ql, key, (uint32_t)j);
+            }
+        }
+        wave_sync();
+    }
+    __syncthreads();
+    topk_flush(a, T, q0, qcount, wave, nwaves, chunk);
+}
+
+// ---------------------------------------------------------------------------- float64 producer
+
+// Work-group: 256 threads; qb queries (64 or 16) x (4096 / qb) rows per step, 4 x 4 pairs per
+// thread, the dimension in slabs of 16 staged in LDS as float64; each pair's sum runs over the
+// dimension in order.
+__global__ __launch_bounds__(256) void exact_f64_kernel(ExactArgs a, int qb) {
+#pragma clang fp contract(off)
+    extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+    constexpr int S = 16;
+    const int qgroups = qb / 4, rgroups = 256 / qgroups, step = 4 * rgroups;
+    double *qs = (double *)lds;  // [qb][S]
+    double *xs = qs + qb * S;    // [step][S]
+    TopK T = topk_carve(lds + (size_t)(qb + step) * S * 8, qb, a.cb, a.k);
+    const int t = threadIdx.x, wave = t >> 6;
+    const int q0 = blockIdx.x * qb;
+    const int chunk = blockIdx.y;
+    const int qcount = min(qb, a.nq - q0);
+    for (int q = t; q < qb; q += 256) {
+        T.cnt[q] = 0;
+        T.tk[q] = kInfKey;
+        T.ti[q] = kInfId;
+    }
+    const int qg = t / rgroups, rg = t % rgroups;
+    const int64_t r_begin = (int64_t)chunk * a.chunk;
+    const int64_t r_end = min(a.n, r_begin + a.chunk);
+    __syncthreads();
+    for (int64_t r0 = r_begin; r0 < r_end; r0 += step) {
+        // Room for one step (at most `step` inserts per query).
+        for (int q = wave; q < qcount; q += 4)
+            if (T.cnt[q] > T.cb - step) topk_compact(T, q);
+        __syncthreads();
+        double acc[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+        for (int s0 = 0; s0 < a.dim; s0 += S) {
+            for (int e = t; e < qb * S; e += 256) {
+                const int row = e / S, s = e % S, dd = s0 + s;
+                const int qi = q0 + row;
+                double qv = 0.0;
+                if (dd < a.dim && qi < a.nq)
+                    qv = a.qn ? a.qn[(size_t)qi * a.dim + dd] : (double)a.q32[(size_t)qi * a.dim + dd];
+                qs[row * S + s] = qv;
+            }
+            for (int e = t; e < step * S; e += 256) {
+                const int row = e / S, s = e % S, dd = s0 + s;
+                const int64_t xj = r0 + row;
+                double xv = 0.0;
+                if (dd < a.dim && xj < r_end)
+                    xv = a.xn ? a.xn[(size_t)xj * a.dim + dd] : (double)a.x32[(size_t)xj * a.dim + dd];
+                xs[row * S + s] = xv;
+            }
+            __syncthreads();
+            const int slab = min(S, a.dim - s0);
+            for (int s = 0; s < slab; ++s) {
+                double qv[4], xv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) qv[u] = qs[(qg * 4 + u) * S + s];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) xv[v] = xs[(rg * 4 + v) * S + s];
+                if (a.metric == kL2) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const double d = xv[v] - qv[u];
+                            acc[u][v] = acc[u][v] + d * d;
+                        }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) acc[u][v] = acc[u][v] + xv[v] * qv[u];
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int ql = qg * 4 + u;
+            if (q0 + ql >= a.nq) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t j = r0 + rg * 4 + v;
+                if (j >= r_end || !row_allowed(a, j)) continue;
+                const double d = a.metric == kL2 ? acc[u][v] : (a.metric == kIP ? -acc[u][v] : 1.0 - acc[u][v]);
+                topk_insert(T, ql, key_of_double(d), (uint32_t)j);
+            }
+        }
+        __syncthreads();
+    }
+    topk_flush(a, T, q0, qcount, wave, 4, chunk);
+}
+
+// ---------------------------------------------------------------------------- merge
+
+__global__ __launch_bounds__(64) void exact_merge_kernel(const uint64_t *pk, const uint32_t *pi,
+                                                         int nchunks, int k, int p2, int exact_int,
+                                                         int metric, int64_t *ids, double *dists) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+    uint64_t *key = (uint64_t *)lds;
+    uint32_t *id = (uint32_t *)(lds + (size_t)p2 * 8);
+    const int q = blockIdx.x, lane = lane_id();
+    const int m = nchunks * k;
+    for (int t = lane; t < m; t += 64) {
+        key[t] = pk[(size_t)q * m + t];
+        id[t] = pi[(size_t)q * m + t];
+    }
+    wave_sync();
+    bitonic_sort(key, id, m, p2);
+    for (int t = lane; t < k; t += 64) {
+        const bool ok = key[t] != kInfKey;
+        ids[(size_t)q * k + t] = ok ? (int64_t)id[t] : -1;
+        dists[(size_t)q * k + t] =
+            !ok ? __builtin_huge_val() : (exact_int ? double_of_exact(key[t], metric) : double_of_key(key[t], metric));
+    }
+}
+
+// Row norms for cosine (float64, left to right) and the normalised float64 copy.
+__global__ void normalise_kernel(const float *x, int64_t rows, int dim, double *out) {
+#pragma clang fp contract(off)
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    double s = 0.0;
+    for (int i = 0; i < dim; ++i) {
+        const double v = (double)x[(size_t)r * dim + i];
+        s = s + v * v;
+    }
+    const double nrm = sqrt(s);
+    for (int i = 0; i < dim; ++i) out[(size_t)r * dim + i] = (double)x[(size_t)r * dim + i] / nrm;
+}
+
+// 8-bit rows -> shifted int8 (zero padded to dimp) plus sum of squares and plain sum.
+__global__ void shift_i8_kernel(const float *x, int64_t rows, int dim, int dimp, int8_t *out,
+                                int32_t *sq, int32_t *sum) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    int32_t s2 = 0, s1 = 0;
+    for (int i = 0; i < dimp; ++i) {
+        int v = i < dim ? (int)x[(size_t)r * dim + i] - 128 : 0;
+        out[(size_t)r * dimp + i] = (int8_t)v;
+        s2 += v * v;
+        s1 += v;
+    }
+    sq[r] = s2;
+    sum[r] = s1;
+}
+
+}  // namespace hm
 
 using namespace hm;
+
+namespace {
+
+bool u8_values(const float *v, size_t count) {
+    for (size_t i = 0; i < count; ++i) {
+        const float x = v[i];
+        if (!(x >= 0.0f && x <= 255.0f) || x != floorf(x)) return false;
+    }
+    return true;
+}
+
+template <class T>
+struct Scoped {
+    T *p = nullptr;
+    hipError_t alloc(size_t count) { return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T)); }
+    ~Scoped() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
 
 extern "C" int hm_exact(const float *data, int64_t n, int32_t dim, int32_t metric,
                         const float *queries, int64_t nq, int32_t k, const uint8_t *allowed,
                         int device, int64_t *ids, double *dists) {
-    (void)data; (void)n; (void)dim; (void)metric; (void)queries; (void)nq; (void)k;
-    (void)allowed; (void)device; (void)ids; (void)dists;
-    return invalid("hm_exact: the matrix-core exact search is not in this build yet");
+    if (!data || !queries || !ids || !dists) return invalid("null argument");
+    if (n < 1 || dim < 1) return invalid("data must be non-empty with at least one dimension");
+    if (k < 1 || k > n) return invalid("k out of range");
+    if (metric < 0 || metric > 2) return invalid("unknown metric");
+    if (n >= (int64_t(1) << 31) - 1) return invalid("more than 2^31-2 rows");
+    if (nq == 0) return HM_OK;
+    HM_TRY(hipSetDevice(device), "hipSetDevice");
+    hipDeviceProp_t prop;
+    HM_TRY(hipGetDeviceProperties(&prop, device), "device properties");
+    const bool int8 = metric != HM_COSINE && dim <= 1024 && u8_values(data, (size_t)n * dim) &&
+                      u8_values(queries, (size_t)nq * dim);
+    ExactArgs a;
+    memset(&a, 0, sizeof(a));
+    a.n = n;
+    a.dim = dim;
+    a.metric = metric;
+    a.k = k;
+    a.nq = (int)nq;
+    // Queries per work-group: as many as the per-query buffers allow in LDS.
+    int qb = 0, cb = 0, lds = 0;
+    for (int cand : int8 ? std::vector<int>{64, 32} : std::vector<int>{64, 16}) {
+        const int step_rows = int8 ? 32 : 4096 / cand;
+        int c = 1;
+        while (c < k + step_rows) c <<= 1;
+        const int need = cand * c * 12 + cand * 4 + cand * 12 + 8 +
+                         (int8 ? 0 : (cand + step_rows) * 16 * 8);
+        if (need <= 160 * 1024) {
+            qb = cand;
+            cb = c;
+            lds = need;
+            break;
+        }
+    }
+    if (!qb) return invalid("k too large for the exact search");
+    a.cb = cb;
+    const int qblocks = (int)((nq + qb - 1) / qb);
+    int nchunks = (int)std::max<int64_t>(1, std::min<int64_t>((4 * prop.multiProcessorCount + qblocks - 1) / qblocks,
+                                                              (n + 2047) / 2048));
+    nchunks = std::max(1, std::min(nchunks, 8192 / k));
+    a.chunk = round_up((n + nchunks - 1) / nchunks, 64);
+    nchunks = (int)((n + a.chunk - 1) / a.chunk);
+    a.nchunks = nchunks;
+    const int m = nchunks * k;
+    int p2 = 1;
+    while (p2 < m) p2 <<= 1;
+    if (p2 * 12 > 160 * 1024) return invalid("k too large for the merge");
+
+    Scoped<float> dx, dq;
+    Scoped<double> dxn, dqn;
+    Scoped<int8_t> x8, q8;
+    Scoped<int32_t> xsq, xsum, qsq, qsum;
+    Scoped<uint32_t> bits;
+    Scoped<uint64_t> pk;
+    Scoped<uint32_t> pi;
+    Scoped<int64_t> oid;
+    Scoped<double> od;
+    hipStream_t s = nullptr;
+    HM_TRY(dx.alloc((size_t)n * dim), "alloc");
+    HM_TRY(dq.alloc((size_t)nq * dim), "alloc");
+    HM_TRY(hipMemcpy(dx.p, data, (size_t)n * dim * 4, hipMemcpyHostToDevice), "copy data");
+    HM_TRY(hipMemcpy(dq.p, queries, (size_t)nq * dim * 4, hipMemcpyHostToDevice), "copy queries");
+    if (allowed) {
+        std::vector<uint32_t> b((size_t)(n + 31) / 32, 0u);
+        for (int64_t i = 0; i < n; ++i)
+            if (allowed[i]) b[i / 32] |= 1u << (i % 32);
+        HM_TRY(bits.alloc(b.size()), "alloc");
+        HM_TRY(hipMemcpy(bits.p, b.data(), b.size() * 4, hipMemcpyHostToDevice), "copy mask");
+        a.allowed = bits.p;
+    }
+    HM_TRY(pk.alloc((size_t)nq * m), "alloc");
+    HM_TRY(pi.alloc((size_t)nq * m), "alloc");
+    HM_TRY(oid.alloc((size_t)nq * k), "alloc");
+    HM_TRY(od.alloc((size_t)nq * k), "alloc");
+    a.part_key = pk.p;
+    a.part_id = pi.p;
+    if (int8) {
+        a.dimp = (int)round_up(dim, 32);
+        HM_TRY(x8.alloc((size_t)n * a.dimp), "alloc");
+        HM_TRY(q8.alloc((size_t)nq * a.dimp), "alloc");
+        HM_TRY(xsq.alloc(n), "alloc");
+        HM_TRY(xsum.alloc(n), "alloc");
+        HM_TRY(qsq.alloc(nq), "alloc");
+        HM_TRY(qsum.alloc(nq), "alloc");
+        hipLaunchKernelGGL(shift_i8_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dx.p, n,
+                           dim, a.dimp, x8.p, xsq.p, xsum.p);
+        hipLaunchKernelGGL(shift_i8_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, dq.p,
+                           nq, dim, a.dimp, q8.p, qsq.p, qsum.p);
+        a.x8 = x8.p;
+        a.q8 = q8.p;
+        a.xsq = xsq.p;
+        a.xsum = xsum.p;
+        a.qsq = qsq.p;
+        a.qsum = qsum.p;
+        const int nch = a.dimp / 32;
+        void (*fn)(ExactArgs) = nch <= 4    ? exact_i8_kernel<4>
+                                : nch <= 8  ? exact_i8_kernel<8>
+                                : nch <= 16 ? exact_i8_kernel<16>
+                                            : exact_i8_kernel<32>;
+        if (lds > 64 * 1024)
+            HM_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds),
+                   "attribute");
+        hipLaunchKernelGGL(fn, dim3(qblocks, nchunks), dim3(2 * qb), lds, s, a);
+    } else {
+        a.x32 = dx.p;
+        a.q32 = dq.p;
+        if (metric == HM_COSINE) {
+            HM_TRY(dxn.alloc((size_t)n * dim), "alloc");
+            HM_TRY(dqn.alloc((size_t)nq * dim), "alloc");
+            hipLaunchKernelGGL(normalise_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dx.p,
+                               n, dim, dxn.p);
+            hipLaunchKernelGGL(normalise_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, dq.p,
+                               nq, dim, dqn.p);
+            a.xn = dxn.p;
+            a.qn = dqn.p;
+        }
+        if (lds > 64 * 1024)
+            HM_TRY(hipFuncSetAttribute((const void *)exact_f64_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds),
+                   "attribute");
+        hipLaunchKernelGGL(exact_f64_kernel, dim3(qblocks, nchunks), dim3(256), lds, s, a, qb);
+    }
+    HM_TRY(hipGetLastError(), "exact launch");
+    const int mlds = p2 * 12;
+    if (mlds > 64 * 1024)
+        HM_TRY(hipFuncSetAttribute((const void *)exact_merge_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, mlds),
+               "attribute");
+    hipLaunchKernelGGL(exact_merge_kernel, dim3((unsigned)nq), dim3(64), mlds, s, pk.p, pi.p, nchunks,
+                       k, p2, int8 ? 1 : 0, metric, oid.p, od.p);
+    HM_TRY(hipGetLastError(), "merge launch");
+    HM_TRY(hipMemcpy(ids, oid.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost), "copy ids");
+    HM_TRY(hipMemcpy(dists, od.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost), "copy dists");
+    return HM_OK;
 }

@@ -1,0 +1,55 @@
+"""CPU checks of the C ABI library and the host graph builder (no GPU calls)."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import golden_io
+from hnsw_amd import _native
+from hnsw_amd.engine import MI355XEngine
+from hnsw_amd.persistence import GraphState, diff_states
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "hnsw_mi355x.h"
+
+
+def test_library_exports_every_declared_symbol() -> None:
+    declared = set(re.findall(r"^\S.*?\b(hm_\w+)\(", HEADER.read_text(), flags=re.M))
+    assert declared == set(_native.SYMBOLS)
+    lib = ctypes.CDLL(str(_native.LIB_PATH))
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+@pytest.mark.parametrize("metric", ["l2", "ip"])
+@pytest.mark.parametrize("tag", ["simple", "heuristic", "extend", "keep", "extend_keep"])
+def test_builder_reproduces_reference_graphs(metric: str, tag: str) -> None:
+    fx = golden_io.load("builder_small_int")
+    cfg = {"simple": ("simple", False, False), "heuristic": ("heuristic", False, False),
+           "extend": ("heuristic", True, False), "keep": ("heuristic", False, True),
+           "extend_keep": ("heuristic", True, True)}[tag]
+    pre = f"{metric}_{tag}_"
+    levels, indptr, indices = golden_io.csr_layers(fx, pre)
+    data = fx["data"].astype(np.float32)
+    eng = MI355XEngine(5, metric, 5, 30, *cfg)
+    eng.insert(data[:300], levels[:300])  # two calls: ids continue, as chunked adds do
+    eng.insert(data[300:], levels[300:])
+    entry, top = (int(v) for v in fx[pre + "entry"])
+    ref = GraphState(data, levels, entry, top, tuple(indptr), tuple(indices))
+    assert diff_states(eng.export_state(), ref) == []
+
+
+@pytest.mark.parametrize("name", ["search_u8_l2", "search_ties_l2", "search_ties_ip",
+                                  "search_float_l2", "search_float_cosine"])
+def test_builder_reproduces_search_fixture_graphs(name: str) -> None:
+    fx, arrays, entry, top, metric, M = golden_io.graph(name)
+    eng = MI355XEngine(arrays["vectors"].shape[1], metric, M, int(fx["params"][1]),
+                       "heuristic", False, False)
+    eng.insert(arrays["vectors"], arrays["levels"])
+    ref = GraphState(arrays["vectors"], arrays["levels"], entry, top,
+                     *[tuple(x) for x in golden_io.csr_layers(fx)[1:]])
+    assert diff_states(eng.export_state(), ref) == []
