@@ -14,7 +14,7 @@
 
 namespace hm {
 
-int search_slots(int mode, bool filt, int lds, int device_cus);
+int search_slots(int mode, bool filt, bool fast, int lds, int device_cus);
 hipError_t search_launch(int mode, bool filt, const SearchArgs &a, int blocks, hipStream_t s);
 
 static thread_local std::string g_error;
@@ -55,16 +55,17 @@ struct Batch {
 constexpr int64_t kDefaultSpill = 16384;
 
 // Capacities of the on-chip state for a batch whose largest max(ef, k) is max_ef.
-void plan(SearchArgs &a, int mode, const hm_index *h, int max_ef) {
+void plan(SearchArgs &a, int mode, const hm_index *h, int max_ef, bool fast) {
     max_ef = std::max(1, max_ef);
-    a.r_cap = (int)round_up(max_ef, 8);
-    a.c_cap = 1 << std::min(12, std::max(6, ceil_log2(2 * (int64_t)max_ef)));
+    a.fast = fast ? 1 : 0;
+    a.r_cap = fast ? 64 : (int)round_up(max_ef, 8);
+    a.c_cap = fast ? 64 : 1 << std::min(12, std::max(6, ceil_log2(2 * (int64_t)max_ef)));
     a.h_log2 = std::min(12, std::max(10, ceil_log2((int64_t)max_ef * 64)));
     a.lds_per_wave = lds_bytes_per_wave(mode, h->dim, h->dimp, a.r_cap, a.c_cap, a.h_log2);
 }
 
 int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, int max_slots,
-           hipStream_t s) {
+           hipStream_t s, bool allow_fast = true) {
     SearchArgs a;
     memset(&a, 0, sizeof(a));
     a.links0 = h->links0.p;
@@ -97,9 +98,9 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, 
     a.out_dc = b.dc;
     a.out_hops = b.hops;
     a.out_fail = b.fail;
-    plan(a, mode, h, b.max_ef);
+    plan(a, mode, h, b.max_ef, allow_fast && !filt && b.max_ef <= 64 && b.kstride <= 64);
     if (a.lds_per_wave > 160 * 1024) return invalid("ef too large for the on-chip search state");
-    int slots = search_slots(mode, filt, a.lds_per_wave, h->cus);
+    int slots = search_slots(mode, filt, a.fast != 0, a.lds_per_wave, h->cus);
     slots = (int)std::min<int64_t>(slots, std::max<int64_t>(1, b.nq));
     if (max_slots > 0) slots = std::min(slots, max_slots);
     a.slots = slots;
@@ -228,7 +229,7 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
     std::vector<int64_t> redo;
     for (int64_t i = 0; i < nq; ++i)
         if (failv[i]) redo.push_back(i);
-    if (!redo.empty() && h->n > kDefaultSpill) {
+    if (!redo.empty()) {
         for (int64_t i : redo) {
             Batch one = b;
             one.nq = 1;
@@ -242,15 +243,13 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
             one.dc = b.dc + i;
             one.hops = b.hops + i;
             one.fail = b.fail + i;
-            st = launch(h, mode, filt, one, h->n, 1, s);
+            st = launch(h, mode, filt, one, h->n, 1, s, false);
             if (st) return st;
         }
         HM_TRY(hipMemcpyAsync(failv.data(), h->fail.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
         HM_TRY(hipStreamSynchronize(s), "search");
         for (int64_t i = 0; i < nq; ++i)
             if (failv[i]) return invalid("candidate spill overflow");
-    } else if (!redo.empty()) {
-        return invalid("candidate spill overflow");
     }
     HM_TRY(hipMemcpyAsync(ids, h->ids.p, on * 8, hipMemcpyDeviceToHost, s), "copy ids");
     HM_TRY(hipMemcpyAsync(dists, h->dists.p, on * 8, hipMemcpyDeviceToHost, s), "copy dists");

@@ -662,9 +662,170 @@ HM_DEV bool layer0(const SearchArgs &a, const Lds &L, Visited &V, Query &Q, int 
     return true;
 }
 
+
+// ---------------------------------------------------------------------------- layer 0, register beam
+
+// Unfiltered layer 0 with ef <= 64, state in registers. Lane i holds R[i] (sorted (key, id));
+// a wave-uniform 64-bit mask marks the entries already expanded. In the unfiltered search every
+// candidate that can still be expanded is either an unexpanded entry of R or an entry evicted
+// from a full R whose key equals the current worst key (anything evicted with a larger key can
+// never be popped before the search ends): the first are popped in R order, then the evicted
+// ties in id order (they were evicted as the largest tuples, so their ids exceed every R entry
+// with that key). When the worst key drops, the ties die. The search ends when no candidate is
+// left, which is when the reference's first pop beyond the worst would break.
+HM_DEV uint64_t readlane_u64(uint64_t v, int j) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, j);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), j);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+HM_DEV uint32_t readlane_u32(uint32_t v, int j) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, j);
+}
+
+template <int MODE>
+HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
+                        uint64_t entry_k, uint32_t entry_id, uint64_t &rk, uint32_t &ri,
+                        int &rn) {
+    const int lane = lane_id();
+    const int ef = Q.ef;
+    constexpr int kTies = 64;
+    uint64_t *const tk = L.ck[0];  // evicted ties, sorted by id, [th, tn)
+    uint32_t *const ti = L.ci[0];
+    visited_reset(a, L, V, a.h_log2);
+    if (lane == 0) visit(L, V, entry_id);
+    V.count = 1;
+    wave_sync();
+    rk = lane == 0 ? entry_k : kInfKey;
+    ri = lane == 0 ? entry_id : kInfId;
+    rn = 1;
+    uint64_t expd = 0;
+    int th = 0, tn = 0;
+    for (;;) {
+        const uint64_t valid = rn >= 64 ? ~0ull : ((1ull << rn) - 1ull);
+        const uint64_t live = valid & ~expd;
+        uint32_t cid;
+        if (live) {
+            const int p = __ffsll((long long)live) - 1;
+            cid = readlane_u32(ri, p);
+            expd |= 1ull << p;
+        } else if (th < tn) {
+            cid = ti[th];
+            th += 1;
+        } else {
+            break;
+        }
+        Q.hops += 1;
+        for (int base = 0; base < a.w0; base += 64) {
+            const int j = base + lane;
+            const int32_t nb = j < a.w0 ? a.links0[(size_t)cid * a.w0 + j] : -1;
+            visited_make_room(L, V, min(64, a.w0 - base));
+            const bool isnew = nb >= 0 && visit(L, V, (uint32_t)nb);
+            const uint64_t nm = ballot(isnew);
+            const int nnew = popc(nm);
+            if (nnew == 0) continue;
+            if (isnew) L.nbi[rank_below(nm)] = (uint32_t)nb;
+            wave_sync();
+            V.count += nnew;
+            const uint64_t ki = distances<MODE>(a, L, nnew, Q.qq);
+            const uint32_t ii = lane < nnew ? L.nbi[lane] : kInfId;
+            Q.dcount += nnew;
+            // accept_i <=> #{r in R: d_r <= d_i} + #{j < i new: d_j <= d_i} < ef
+            int c_r = 0, c_n = 0;
+            for (int q = 0; q < nnew; ++q) {
+                const uint64_t kq = readlane_u64(ki, q);
+                const int cnt = popc(ballot(lane < rn && rk <= kq));
+                if (lane == q) c_r = cnt;
+                if (q < lane && kq <= ki) c_n += 1;
+            }
+            const bool acc = lane < nnew && c_r + c_n < ef;
+            const uint64_t am = ballot(acc);
+            if (am == 0) continue;
+            const int na = popc(am);
+            // Merge positions: R entry -> lane + #{accepted below it}; accepted entry ->
+            // #{R below it} + #{accepted below it}.
+            int pos_r = lane, r_below = 0, a_below = 0;
+            for (uint64_t m = am; m; m &= m - 1ull) {
+                const int q = __ffsll((long long)m) - 1;
+                const uint64_t kq = readlane_u64(ki, q);
+                const uint32_t iq = readlane_u32(ii, q);
+                if (less(kq, iq, rk, ri)) pos_r += 1;
+                const int below = popc(ballot(lane < rn && less(rk, ri, kq, iq)));
+                if (lane == q) r_below = below;
+                if (acc && less(kq, iq, ki, ii)) a_below += 1;
+            }
+            const int pos_a = r_below + a_below;
+            const int total = rn + na;
+            const int nrn = min(ef, total);
+            const bool rflag = (expd >> lane) & 1ull;
+            if (lane < rn && pos_r < ef) {
+                L.rk[0][pos_r] = rk;
+                L.ri[0][pos_r] = ri;
+                L.nbal[pos_r] = rflag ? 1u : 0u;
+            }
+            if (acc && pos_a < ef) {
+                L.rk[0][pos_a] = ki;
+                L.ri[0][pos_a] = ii;
+                L.nbal[pos_a] = 0u;
+            }
+            wave_sync();
+            const uint64_t nrk = lane < nrn ? L.rk[0][lane] : kInfKey;
+            const uint32_t nri = lane < nrn ? L.ri[0][lane] : kInfId;
+            const bool nflag = lane < nrn && L.nbal[lane] != 0u;
+            if (total > ef) {
+                // Evictions: keep the unexpanded ones whose key equals the new worst.
+                const uint64_t nwk = readlane_u64(nrk, ef - 1);
+                const bool was_full = rn == ef;
+                const uint64_t owk = was_full ? readlane_u64(rk, ef - 1) : kInfKey;
+                if (!was_full || nwk < owk) {
+                    th = 0;
+                    tn = 0;
+                }
+                const bool er = lane < rn && pos_r >= ef && !rflag && rk == nwk;
+                const bool ea = acc && pos_a >= ef && ki == nwk;
+                const uint64_t em = ballot(er || ea);
+                if (em) {
+                    const int ne = popc(em);
+                    // compact the live ties to the front, then append the new ones and sort
+                    const int live_t = tn - th;
+                    if (live_t + ne > kTies) return false;
+                    uint64_t k0 = 0;
+                    uint32_t i0 = 0;
+                    if (lane < live_t) {  // live_t <= kTies = 64: one entry per lane
+                        k0 = tk[th + lane];
+                        i0 = ti[th + lane];
+                    }
+                    wave_sync();
+                    if (lane < live_t) {
+                        tk[lane] = k0;
+                        ti[lane] = i0;
+                    }
+                    wave_sync();
+                    if (er || ea) {
+                        const int d = live_t + rank_below(em);
+                        tk[d] = er ? rk : ki;
+                        ti[d] = er ? ri : ii;
+                    }
+                    th = 0;
+                    tn = live_t + ne;
+                    int p2 = 1;
+                    while (p2 < tn) p2 <<= 1;
+                    bitonic_sort(tk, ti, tn, p2);
+                }
+            }
+            wave_sync();
+            rk = nrk;
+            ri = nri;
+            expd = ballot(nflag);
+            rn = nrn;
+        }
+    }
+    return true;
+}
+
 // ---------------------------------------------------------------------------- one query
 
-template <int MODE, bool FILT>
+template <int MODE, bool FILT, bool FAST>
 HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int slot) {
     const int lane = lane_id();
     Query Q;
@@ -706,20 +867,34 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
     for (int layer = a.max_level; layer > 0; --layer)
         greedy_layer<MODE>(a, L, V, Q, layer, cur_k, cur_id);
 
-    int rb = 0, rn = 0;
-    const bool ok = layer0<MODE, FILT>(a, L, V, Q, slot, cur_k, cur_id, rb, rn);
-    if (!ok) rn = 0;
-
     int64_t *oi = a.out_ids + (size_t)qi * a.kstride;
     double *od = a.out_d + (size_t)qi * a.kstride;
-    const int take = min(Q.k, rn);
-    for (int i = lane; i < a.kstride; i += 64) {
-        if (i < take) {
-            oi[i] = (int64_t)L.ri[rb][i];
-            od[i] = decode<MODE>(L.rk[rb][i], a.metric);
-        } else {
-            oi[i] = -1;
-            od[i] = __builtin_huge_val();
+    int rn = 0;
+    bool ok;
+    if constexpr (FAST) {
+        uint64_t rk;
+        uint32_t ri;
+        ok = layer0_fast<MODE>(a, L, V, Q, cur_k, cur_id, rk, ri, rn);
+        if (!ok) rn = 0;
+        const int take = min(Q.k, rn);
+        for (int i = lane; i < a.kstride; i += 64) {
+            const bool own = i < take;  // lane i holds R[i] (kstride <= 64 on this path)
+            oi[i] = own ? (int64_t)ri : -1;
+            od[i] = own ? decode<MODE>(rk, a.metric) : __builtin_huge_val();
+        }
+    } else {
+        int rb = 0;
+        ok = layer0<MODE, FILT>(a, L, V, Q, slot, cur_k, cur_id, rb, rn);
+        if (!ok) rn = 0;
+        const int take = min(Q.k, rn);
+        for (int i = lane; i < a.kstride; i += 64) {
+            if (i < take) {
+                oi[i] = (int64_t)L.ri[rb][i];
+                od[i] = decode<MODE>(L.rk[rb][i], a.metric);
+            } else {
+                oi[i] = -1;
+                od[i] = __builtin_huge_val();
+            }
         }
     }
     if (lane == 0) {
@@ -730,7 +905,7 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
     wave_sync();
 }
 
-template <int MODE, bool FILT>
+template <int MODE, bool FILT, bool FAST>
 __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
     extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
     const int slot = blockIdx.x;
@@ -746,7 +921,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
         if (lane_id() == 0) qi = atomicAdd(a.queue, 1);
         qi = uniform(shfl(qi, 0));
         if (qi >= a.nq) break;
-        run_query<MODE, FILT>(a, L, V, qi, slot);
+        run_query<MODE, FILT, FAST>(a, L, V, qi, slot);
     }
     // Leave the bitset clean for the next launch.
     if (V.dirty) {
@@ -756,11 +931,14 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
 
 __global__ void reset_queue_kernel(int32_t *q) { q[0] = 0; }
 
-template __global__ void search_kernel<kExactU8, false>(SearchArgs);
-template __global__ void search_kernel<kExactU8, true>(SearchArgs);
-template __global__ void search_kernel<kSeqF32, false>(SearchArgs);
-template __global__ void search_kernel<kSeqF32, true>(SearchArgs);
-template __global__ void search_kernel<kSeqU8, false>(SearchArgs);
+template __global__ void search_kernel<kExactU8, false, false>(SearchArgs);
+template __global__ void search_kernel<kExactU8, true, false>(SearchArgs);
+template __global__ void search_kernel<kSeqF32, false, false>(SearchArgs);
+template __global__ void search_kernel<kSeqF32, true, false>(SearchArgs);
+template __global__ void search_kernel<kSeqU8, false, false>(SearchArgs);
+template __global__ void search_kernel<kExactU8, false, true>(SearchArgs);
+template __global__ void search_kernel<kSeqF32, false, true>(SearchArgs);
+template __global__ void search_kernel<kSeqU8, false, true>(SearchArgs);
 
 }  // namespace hm
 
@@ -770,15 +948,22 @@ namespace hm {
 
 using KernelFn = void (*)(SearchArgs);
 
-static KernelFn pick(int mode, bool filt) {
-    if (mode == kExactU8) return filt ? search_kernel<kExactU8, true> : search_kernel<kExactU8, false>;
-    if (mode == kSeqF32) return filt ? search_kernel<kSeqF32, true> : search_kernel<kSeqF32, false>;
-    return search_kernel<kSeqU8, false>;
+static KernelFn pick(int mode, bool filt, bool fast) {
+    if (fast && !filt) {
+        if (mode == kExactU8) return search_kernel<kExactU8, false, true>;
+        if (mode == kSeqF32) return search_kernel<kSeqF32, false, true>;
+        return search_kernel<kSeqU8, false, true>;
+    }
+    if (mode == kExactU8)
+        return filt ? search_kernel<kExactU8, true, false> : search_kernel<kExactU8, false, false>;
+    if (mode == kSeqF32)
+        return filt ? search_kernel<kSeqF32, true, false> : search_kernel<kSeqF32, false, false>;
+    return search_kernel<kSeqU8, false, false>;
 }
 
 // Blocks (one wave each) that fit on the device at once for this LDS size.
-int search_slots(int mode, bool filt, int lds, int device_cus) {
-    KernelFn fn = pick(mode, filt);
+int search_slots(int mode, bool filt, bool fast, int lds, int device_cus) {
+    KernelFn fn = pick(mode, filt, fast);
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     int per_cu = 0;
@@ -790,7 +975,7 @@ int search_slots(int mode, bool filt, int lds, int device_cus) {
 }
 
 hipError_t search_launch(int mode, bool filt, const SearchArgs &a, int blocks, hipStream_t s) {
-    KernelFn fn = pick(mode, filt);
+    KernelFn fn = pick(mode, filt, a.fast != 0);
     if (a.lds_per_wave > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)fn,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,

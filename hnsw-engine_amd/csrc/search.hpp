@@ -58,6 +58,7 @@ struct SearchArgs {
     int r_cap, c_cap, h_log2;
     int lds_per_wave;
     int slots;
+    int fast;  // 1: unfiltered, every ef <= 64 -> the register-beam layer-0 kernel
 };
 
 // Bytes of LDS one wave needs for the given capacities.

@@ -151,3 +151,20 @@ def test_rerank_matches_oracle(built_u8) -> None:
     want = oracle.rerank_batch(data, "l2", queries, cand, 10, threads=8)
     np.testing.assert_array_equal(got[0], want[0])
     np.testing.assert_array_equal(got[1], want[1])
+
+
+def test_massive_ties_overflow_the_register_beam_and_rerun() -> None:
+    """16 distinct points repeated 200 times: the evicted ties at the worst distance outgrow
+    the register-beam's tie list, so those queries rerun on the general kernel."""
+    rng = np.random.default_rng(12)
+    base = rng.integers(0, 2, size=(3200, 4)).astype(np.float32)
+    q = rng.integers(0, 2, size=(80, 4)).astype(np.float32)
+    levels = np.floor(-np.log(1.0 - rng.random(3200)) / np.log(8)).astype(np.int64)
+    eng = MI355XEngine(4, "l2", 8, 40, "heuristic", False, False)
+    eng.insert(base, levels)
+    for ef, k in ((10, 10), (40, 5), (64, 64)):
+        print(f"case ties ef={ef} k={k}", flush=True)
+        got = eng.search_many(q, k, ef)
+        want = oracle.search_batch(eng.native_arrays(), eng.entry_point, eng.max_level, "l2",
+                                   q, k, ef, threads=8)
+        _same(got, want, f"ties ef={ef}")
