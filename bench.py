@@ -5,12 +5,16 @@ Workload (``hnsw_amd.workloads.clustered_1m``): 1,000,000 x 128 float32 vectors 
 values 0..255 from 1,000 Gaussian clusters, 10,000 queries, L2, M = 16, ef_construction = 200;
 the graph is built by sequential insertion on the host (cached under ``.graph_cache/``).
 
-Steps: exact top-10 ground truth with ``exact_search`` (GPU); a sweep of ef over
-{16, 32, 64, 128, 256, 512} picks the smallest ef whose recall@10 >= 0.95; then W untimed and
-K timed steps, a step being one batch search of all 10,000 queries with queries and outputs
-already in HBM. The JSON line carries the roofline of the search kernel (algorithmic bytes per
-launch over the launch's average duration, HIP events on the launch stream) and the CPU
-baseline (the C oracle, 1 thread and all cores, on a bounded query sample, ids checked equal).
+Steps: exact top-10 ground truth with ``exact_search`` (GPU). Plain beam: a sweep of ef picks
+the smallest ef whose recall@10 >= 0.95. Learned stop: traces of the plain search on 10,000
+training queries (disjoint from the measured ones) label each layer-0 expansion as final or not,
+the stop MLP is trained on the GPU (<= 200,000 rows), and a sweep of (ef, threshold) picks the
+fastest setting with recall@10 >= 0.95. Then W untimed and K timed steps of the faster variant
+(the learned stop, unless it cannot reach the recall), a step being one batch search of all
+10,000 queries with queries and outputs already in HBM. The JSON line carries the roofline of
+the search kernel (algorithmic bytes per launch over the launch's average duration, HIP events
+on the launch stream) and the CPU baseline (the C oracle at the plain beam's ef, 1 thread and all
+cores, on a bounded query sample, ids checked equal to the GPU's plain results).
 """
 
 from __future__ import annotations
@@ -55,10 +59,14 @@ def main() -> int:
     ap.add_argument("--workload", default="clustered_1m")
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--target-recall", type=float, default=0.95)
-    ap.add_argument("--sweep", default="16,32,64,128,256,512")
+    ap.add_argument("--sweep", default="16,24,32,40,48,56,64,96,128,256,512")
     ap.add_argument("--cpu-seconds", type=float, default=8.0,
                     help="target CPU time of each CPU-baseline leg")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--train-queries", type=int, default=10_000)
+    ap.add_argument("--learned-ef", default="56,64")
+    ap.add_argument("--thresholds", default="0.6,0.7,0.8,0.85,0.9,0.95")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -97,21 +105,21 @@ def main() -> int:
     fail_dev = torch.zeros(nq, dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream()
 
-    def launch(ef: int) -> None:
+    def launch(ef: int, learned: bool = False) -> None:
         opts = _native.SearchOpts()
         opts.nq, opts.k_all, opts.ef_all, opts.kstride, opts.mask_all = nq, k, ef, k, -1
+        opts.learned = int(learned)
         _native.check(lib.hm_search_dev(
             handle, ctypes.c_void_p(q_dev.data_ptr()), ctypes.byref(opts), None, int(exact_u8),
             ctypes.c_void_p(ids_dev.data_ptr()), ctypes.c_void_p(d_dev.data_ptr()),
             ctypes.c_void_p(dc_dev.data_ptr()), ctypes.c_void_p(hops_dev.data_ptr()),
             ctypes.c_void_p(fail_dev.data_ptr()), ctypes.c_void_p(stream.cuda_stream)))
 
-    sweep = []
-    for ef in [int(v) for v in args.sweep.split(",")]:
-        launch(ef)
+    def measure(ef: int, learned: bool = False) -> dict:
+        launch(ef, learned)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        launch(ef)
+        launch(ef, learned)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t
         if int(fail_dev.sum()) != 0:
@@ -120,16 +128,71 @@ def main() -> int:
         rec = recall_at_k(ids, exact_l2(data, queries, ids), true_ids, true_d, k)
         dc = dc_dev.cpu().numpy().astype(np.int64)
         hops = hops_dev.cpu().numpy().astype(np.int64)
-        sweep.append({"ef": ef, "recall": rec["recall"], "recall_ids": rec["recall_ids"],
-                      "qps": nq / dt, "dcount_mean": float(dc.mean()),
-                      "hops_mean": float(hops.mean()), "ids": ids, "dc": dc, "hops": hops})
-        log(f"[bench] ef={ef} recall={rec['recall']:.4f} qps={nq / dt:,.0f} "
-            f"dist/q={dc.mean():.0f} hops/q={hops.mean():.0f}")
-    chosen = next((s for s in sweep if s["recall"] >= args.target_recall), sweep[-1])
+        return {"ef": ef, "recall": rec["recall"], "recall_ids": rec["recall_ids"],
+                "qps": nq / dt, "dcount_mean": float(dc.mean()), "hops_mean": float(hops.mean()),
+                "ids": ids, "dc": dc, "hops": hops, "learned": learned}
+
+    sweep = []
+    for ef in [int(v) for v in args.sweep.split(",")]:
+        sweep.append(measure(ef))
+        r = sweep[-1]
+        log(f"[bench] plain ef={ef} recall={r['recall']:.4f} qps={r['qps']:,.0f} "
+            f"dist/q={r['dcount_mean']:.0f} hops/q={r['hops_mean']:.0f}")
+    plain = next((s for s in sweep if s["recall"] >= args.target_recall), sweep[-1])
+    chosen = plain
+
+    learned_info = None
+    if not args.no_learned:
+        from hnsw_amd import learned as lrn
+
+        t0 = time.perf_counter()
+        train_q = w.train_queries(args.train_queries)
+        _, t_d = exact_search(data, train_q, k)
+        thr = lrn.recall_thresholds(t_d, k)
+        xs, ys = [], []
+        for ef_l in [int(v) for v in args.learned_ef.split(",")]:
+            trace, length, final = eng.trace(train_q, k, ef_l, thr, steps=256)
+            x, y = lrn.dataset_from_traces(trace, length, final)
+            xs.append(x)
+            ys.append(y)
+        x = np.concatenate(xs)
+        y = np.concatenate(ys)
+        if x.shape[0] > 200_000:
+            pick = np.sort(np.random.default_rng(1).choice(x.shape[0], 200_000, replace=False))
+            x, y = x[pick], y[pick]
+        params, losses = lrn.train(x, y, epochs=12, seed=3)
+        t_train = time.perf_counter() - t0
+        log(f"[bench] stop model: {x.shape[0]} rows ({y.mean():.2f} positive), loss "
+            f"{losses[0]:.4f} -> {losses[-1]:.4f}, {t_train:.1f}s")
+        lsweep = []
+        for ef_l in [int(v) for v in args.learned_ef.split(",")]:
+            for tau in [float(v) for v in args.thresholds.split(",")]:
+                eng.set_stop_model(params, tau)
+                r = measure(ef_l, learned=True)
+                r["threshold"] = tau
+                lsweep.append(r)
+                log(f"[bench] learned ef={ef_l} threshold={tau} recall={r['recall']:.4f} "
+                    f"qps={r['qps']:,.0f} dist/q={r['dcount_mean']:.0f} hops/q={r['hops_mean']:.0f}")
+        ok = [r for r in lsweep if r["recall"] >= args.target_recall]
+        best = max(ok, key=lambda r: r["qps"]) if ok else None
+        learned_best = best
+        learned_info = {
+            "train_rows": int(x.shape[0]), "positive_share": float(y.mean()),
+            "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+            "train_seconds": t_train,
+            "sweep": [{kk: r[kk] for kk in ("ef", "threshold", "recall", "qps", "dcount_mean",
+                                             "hops_mean")} for r in lsweep],
+        }
+        if best is not None and best["qps"] > plain["qps"]:
+            chosen = best
+            eng.set_stop_model(params, best["threshold"])
+        learned_info["best"] = None if learned_best is None else {
+            kk: learned_best[kk] for kk in ("ef", "threshold", "recall", "qps", "dcount_mean")}
     ef = chosen["ef"]
+    use_learned = chosen.get("learned", False)
 
     for _ in range(args.warmup):
-        launch(ef)
+        launch(ef, use_learned)
     torch.cuda.synchronize()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
@@ -138,7 +201,7 @@ def main() -> int:
     t_begin.record(stream)
     for i in range(args.steps):
         starts[i].record(stream)
-        launch(ef)
+        launch(ef, use_learned)
         ends[i].record(stream)
     t_end.record(stream)
     torch.cuda.synchronize()
@@ -160,12 +223,12 @@ def main() -> int:
     tfile = ROOT / "profiles" / f"search_{args.workload}_traffic.json"
     if tfile.is_file():
         tj = json.loads(tfile.read_text())
-        if tj.get("ef") == ef:
+        if tj.get("ef") == ef and bool(tj.get("learned", False)) == bool(use_learned):
             traffic = tj.get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(eng, queries, k, ef, chosen["ids"], args.cpu_seconds)
+        cpu = cpu_baseline(eng, queries, k, plain["ef"], plain["ids"], args.cpu_seconds)
         cpu["ids_match_gpu"] = bool(cpu.pop("match"))
 
     line = {
@@ -184,9 +247,16 @@ def main() -> int:
         "config": {
             "workload": f"{args.workload}: {w.n} x {w.dim} float32, integer values 0..255, "
             f"{w.n_clusters} clusters, {nq} queries, L2, M={w.M}, efC={w.ef_construction}",
-            "k": k, "ef": ef, "recall": chosen["recall"], "variant": "plain beam",
+            "k": k, "ef": ef, "recall": chosen["recall"],
+            "variant": (f"learned stop (threshold {chosen['threshold']})" if use_learned
+                        else "plain beam"),
+            "plain": {"ef": plain["ef"], "recall": plain["recall"], "qps": plain["qps"]},
+            "learned_speedup_vs_plain": (
+                learned_info["best"]["qps"] / plain["qps"]
+                if learned_info and learned_info.get("best") else None),
             "sweep": [{kk: s[kk] for kk in ("ef", "recall", "qps", "dcount_mean", "hops_mean")}
                       for s in sweep],
+            "learned": learned_info,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -14,7 +14,7 @@
 
 namespace hm {
 
-int search_slots(int mode, bool filt, bool fast, int lds, int device_cus);
+int search_slots(int mode, bool filt, bool fast, int aux, int lds, int device_cus);
 hipError_t search_launch(int mode, bool filt, const SearchArgs &a, int blocks, hipStream_t s);
 
 static thread_local std::string g_error;
@@ -50,6 +50,12 @@ struct Batch {
     int64_t *ids = nullptr;
     double *dists = nullptr;
     int32_t *dc = nullptr, *hops = nullptr, *fail = nullptr;
+    bool learned = false;  // use the handle's stop model (register-beam path only)
+    // traces (register-beam path only)
+    float *trace = nullptr;
+    int trace_steps = 0;
+    const double *trace_thr = nullptr;
+    int32_t *trace_len = nullptr, *trace_final = nullptr;
 };
 
 constexpr int64_t kDefaultSpill = 16384;
@@ -58,9 +64,20 @@ constexpr int64_t kDefaultSpill = 16384;
 void plan(SearchArgs &a, int mode, const hm_index *h, int max_ef, bool fast) {
     max_ef = std::max(1, max_ef);
     a.fast = fast ? 1 : 0;
-    a.r_cap = fast ? 64 : (int)round_up(max_ef, 8);
-    a.c_cap = fast ? 64 : 1 << std::min(12, std::max(6, ceil_log2(2 * (int64_t)max_ef)));
-    a.h_log2 = std::min(12, std::max(10, ceil_log2((int64_t)max_ef * 64)));
+    if (fast) {
+        // 2048 slots (8 KB): measured on the 1M x 128 workload for ef 32..64 it beats 4096
+        // slots (fewer waves fit) although ~6 % of queries at ef 48 outgrow it and continue on
+        // the global bitset; 1024 slots overflow for nearly every query.
+        a.h_log2 = h->tune_fast_hlog ? h->tune_fast_hlog : (max_ef <= 8 ? 10 : 11);
+        a.r_cap = 64;
+        a.c_cap = 64;
+        a.lds_per_wave = lds_bytes_fast(mode, h->dim, h->dimp, a.h_log2);
+        return;
+    }
+    a.r_cap = (int)round_up(max_ef, 8);
+    a.c_cap = 1 << std::min(12, std::max(6, ceil_log2(2 * (int64_t)max_ef)));
+    a.h_log2 = h->tune_hlog ? h->tune_hlog
+                            : std::min(12, std::max(10, ceil_log2((int64_t)max_ef * 64)));
     a.lds_per_wave = lds_bytes_per_wave(mode, h->dim, h->dimp, a.r_cap, a.c_cap, a.h_log2);
 }
 
@@ -98,9 +115,27 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, 
     a.out_dc = b.dc;
     a.out_hops = b.hops;
     a.out_fail = b.fail;
-    plan(a, mode, h, b.max_ef, allow_fast && !filt && b.max_ef <= 64 && b.kstride <= 64);
+    const bool fast = allow_fast && !filt && b.max_ef <= 64 && b.kstride <= 64 && mode != kSeqU8;
+    plan(a, mode, h, b.max_ef, fast || (allow_fast && !filt && b.max_ef <= 64 && b.kstride <= 64));
+    if (fast && b.trace) {
+        a.trace = b.trace;
+        a.trace_steps = b.trace_steps;
+        a.trace_thr = b.trace_thr;
+        a.trace_len = b.trace_len;
+        a.trace_final = b.trace_final;
+    } else if (b.trace) {
+        return invalid("traces need an unfiltered batch with ef <= 64");
+    } else if (fast && b.learned && h->stop_on) {
+        a.stop_w = h->stop_w.p;
+        a.stop_logit = h->stop_logit;
+        a.stop_tmin = h->stop_tmin;
+    }
     if (a.lds_per_wave > 160 * 1024) return invalid("ef too large for the on-chip search state");
-    int slots = search_slots(mode, filt, a.fast != 0, a.lds_per_wave, h->cus);
+    const int aux = a.trace ? 1 : (a.stop_w ? 2 : 0);
+    int slots = search_slots(mode, filt, a.fast != 0, aux, a.lds_per_wave, h->cus);
+    if (h->tune_waves_per_cu > 0) slots = std::min(slots, h->tune_waves_per_cu * h->cus);
+    HM_TRY(h->stats.ensure(2, true, s), "stats allocation");
+    a.stats = h->stats.p;
     slots = (int)std::min<int64_t>(slots, std::max<int64_t>(1, b.nq));
     if (max_slots > 0) slots = std::min(slots, max_slots);
     a.slots = slots;
@@ -177,6 +212,7 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
         max_ef = b.ef_all;
     }
     b.max_ef = max_ef;
+    b.learned = o->learned != 0;
     bool filt = false;
     std::vector<uint32_t> bits;
     if (o->mask_pool && o->n_masks > 0 && (o->mask_index || o->mask_all >= 0)) {
@@ -421,8 +457,78 @@ int hm_search_dev(hm_index *h, const float *d_queries, const hm_search_opts *o,
     b.dc = d_dcount ? d_dcount : h->dc.p;
     b.hops = d_hops ? d_hops : h->hops.p;
     b.fail = d_fail ? d_fail : h->fail.p;
+    b.learned = o->learned != 0;
     return launch(h, exact_u8 ? kExactU8 : kSeqF32, filt, b, std::min<int64_t>(h->n, kDefaultSpill),
                   0, s);
+}
+
+int hm_trace_dev(hm_index *h, const float *d_queries, int64_t nq, int32_t k, int32_t ef,
+                 int exact_u8, const double *d_thr, float *d_trace, int32_t steps,
+                 int32_t *d_len, int32_t *d_final, void *stream) {
+    if (!h || !d_queries || !d_thr || !d_trace || !d_len || !d_final) return invalid("null argument");
+    if (k < 1 || ef < 1 || std::max(ef, k) > 64 || steps < 1) return invalid("traces need k <= ef <= 64");
+    if (exact_u8 && !h->exact_u8) return invalid("the index keeps no exact 8-bit copy");
+    if (h->n == 0 || h->entry < 0) return invalid("cannot search an empty index");
+    if (nq == 0) return HM_OK;
+    HM_TRY(hipSetDevice(h->device), "hipSetDevice");
+    HM_TRY(h->ids.ensure((size_t)nq * k), "alloc");
+    HM_TRY(h->dists.ensure((size_t)nq * k), "alloc");
+    HM_TRY(h->dc.ensure(nq), "alloc");
+    HM_TRY(h->hops.ensure(nq), "alloc");
+    HM_TRY(h->fail.ensure(nq), "alloc");
+    Batch b;
+    b.nq = nq;
+    b.q32 = d_queries;
+    b.kstride = k;
+    b.k_all = k;
+    b.ef_all = std::max(ef, k);
+    b.max_ef = b.ef_all;
+    b.ids = h->ids.p;
+    b.dists = h->dists.p;
+    b.dc = h->dc.p;
+    b.hops = h->hops.p;
+    b.fail = h->fail.p;
+    b.trace = d_trace;
+    b.trace_steps = steps;
+    b.trace_thr = d_thr;
+    b.trace_len = d_len;
+    b.trace_final = d_final;
+    return launch(h, exact_u8 ? kExactU8 : kSeqF32, false, b, std::min<int64_t>(h->n, kDefaultSpill),
+                  0, (hipStream_t)stream);
+}
+
+int hm_trace(hm_index *h, const float *queries, int64_t nq, int32_t k, int32_t ef,
+             const double *thr, float *trace, int32_t steps, int32_t *len, int32_t *final_hits) {
+    if (!h || !queries || !thr || !trace || !len || !final_hits) return invalid("null argument");
+    if (nq == 0) return HM_OK;
+    HM_TRY(hipSetDevice(h->device), "hipSetDevice");
+    hipStream_t s = h->stream;
+    const size_t qn = (size_t)nq * h->dim, tn = (size_t)nq * steps * kTraceCols;
+    DevBuf<float> dq, dtr;
+    DevBuf<double> dthr;
+    DevBuf<int32_t> dlen, dfin;
+    struct Free {
+        DevBuf<float> *a, *b;
+        DevBuf<double> *c;
+        DevBuf<int32_t> *d, *e;
+        ~Free() { a->release(); b->release(); c->release(); d->release(); e->release(); }
+    } guard{&dq, &dtr, &dthr, &dlen, &dfin};
+    HM_TRY(dq.ensure(qn), "alloc");
+    HM_TRY(dtr.ensure(tn), "alloc");
+    HM_TRY(dthr.ensure(nq), "alloc");
+    HM_TRY(dlen.ensure(nq), "alloc");
+    HM_TRY(dfin.ensure(nq), "alloc");
+    HM_TRY(hipMemcpyAsync(dq.p, queries, qn * 4, hipMemcpyHostToDevice, s), "copy");
+    HM_TRY(hipMemcpyAsync(dthr.p, thr, nq * 8, hipMemcpyHostToDevice, s), "copy");
+    HM_TRY(hipMemsetAsync(dtr.p, 0, tn * 4, s), "memset");
+    const int exact = h->exact_u8 && hm_queries_exact(queries, (int64_t)qn) ? 1 : 0;
+    int st = hm_trace_dev(h, dq.p, nq, k, ef, exact, dthr.p, dtr.p, steps, dlen.p, dfin.p, s);
+    if (st) return st;
+    HM_TRY(hipMemcpyAsync(trace, dtr.p, tn * 4, hipMemcpyDeviceToHost, s), "copy");
+    HM_TRY(hipMemcpyAsync(len, dlen.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
+    HM_TRY(hipMemcpyAsync(final_hits, dfin.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
+    HM_TRY(hipStreamSynchronize(s), "trace");
+    return HM_OK;
 }
 
 int hm_attach_codes(hm_index *h, const uint8_t *codes) {
@@ -442,6 +548,28 @@ int hm_search_codes(hm_index *h, const double *queries, const hm_search_opts *o,
     if (!h || !queries || !o || !ids || !dists) return invalid("null argument");
     if (!h->has_codes) return invalid("no codes attached");
     return search_host(h, kSeqU8, nullptr, queries, o, ids, dists, dcount, hops);
+}
+
+int hm_set_tuning(hm_index *h, int32_t fast_hash_log2, int32_t hash_log2, int32_t waves_per_cu) {
+    if (!h) return invalid("null argument");
+    if ((fast_hash_log2 && (fast_hash_log2 < 6 || fast_hash_log2 > 14)) ||
+        (hash_log2 && (hash_log2 < 6 || hash_log2 > 14)) || waves_per_cu < 0)
+        return invalid("tuning values out of range");
+    h->tune_fast_hlog = fast_hash_log2;
+    h->tune_hlog = hash_log2;
+    h->tune_waves_per_cu = waves_per_cu;
+    return HM_OK;
+}
+
+int hm_read_stats(hm_index *h, int32_t *out, int reset) {
+    if (!h || !out) return invalid("null argument");
+    HM_TRY(hipSetDevice(h->device), "hipSetDevice");
+    HM_TRY(h->stats.ensure(2, true, h->stream), "stats allocation");
+    HM_TRY(hipStreamSynchronize(h->stream), "sync");
+    HM_TRY(hipDeviceSynchronize(), "sync");
+    HM_TRY(hipMemcpy(out, h->stats.p, 8, hipMemcpyDeviceToHost), "copy stats");
+    if (reset) HM_TRY(hipMemset(h->stats.p, 0, 8), "reset stats");
+    return HM_OK;
 }
 
 const char *hm_last_error(void) { return g_error.c_str(); }

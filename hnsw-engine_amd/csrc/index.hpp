@@ -29,6 +29,15 @@ struct hm_index {
     hm::DevBuf<uint32_t> maskbits;
     hm::DevBuf<int64_t> ids, cand;
     hm::DevBuf<double> dists;
+    // learned stop (hm_set_stop_model)
+    hm::DevBuf<float> stop_w;
+    float stop_logit = 0.f;
+    int stop_tmin = 0;
+    bool stop_on = false;
+    // tuning (0: automatic): visited-hash size of the register-beam and general kernels, waves
+    // per compute unit
+    int tune_fast_hlog = 0, tune_hlog = 0, tune_waves_per_cu = 0;
+    hm::DevBuf<int32_t> stats;  // event counters, see SearchArgs::stats
 
     void release_all() {
         hm::DevBuf<int32_t> *i32[] = {&links0, &links_up, &upper, &queue, &kq,
@@ -46,6 +55,8 @@ struct hm_index {
         ids.release();
         cand.release();
         dists.release();
+        stop_w.release();
+        stats.release();
     }
 };
 

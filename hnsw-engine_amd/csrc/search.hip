@@ -76,6 +76,24 @@ HM_DEV Lds carve(uint8_t *base, const SearchArgs &a, int mode) {
     return L;
 }
 
+// Compact layout of the register-beam kernel (fields it does not use stay null).
+HM_DEV Lds carve_fast(uint8_t *base, const SearchArgs &a, int mode) {
+    auto up = [](int v) { return (v + 15) & ~15; };
+    Lds L = {};
+    uint8_t *p = base;
+    L.qb = p;
+    p += mode == kExactU8 ? up(a.dimp) : up(a.dim * 8);
+    L.rk[0] = (uint64_t *)p; p += 64 * 8;
+    L.ck[0] = (uint64_t *)p; p += 64 * 8;
+    L.ri[0] = (uint32_t *)p; p += 64 * 4;
+    L.ci[0] = (uint32_t *)p; p += 64 * 4;
+    L.nbal = (uint32_t *)p; p += 64 * 4;
+    L.nbi = (uint32_t *)p; p += 64 * 4;
+    L.nbacc = (int32_t *)p; p += 64 * 4;
+    L.hash = (uint32_t *)p;
+    return L;
+}
+
 // Number of entries of sorted (k, i)[0..n) strictly below (bk, bi).
 HM_DEV int lower_count(const uint64_t *k, const uint32_t *id, int n, uint64_t bk, uint32_t bi) {
     int lo = 0, hi = n;
@@ -247,6 +265,7 @@ HM_DEV double decode(uint64_t k, int metric) {
 // ---------------------------------------------------------------------------- visited set
 
 struct Visited {
+    int32_t *stats;
     uint32_t *bits;
     int hlog;    // active hash size is 1 << hlog
     int count;   // ids inserted in this layer
@@ -278,6 +297,7 @@ HM_DEV void visited_make_room(const Lds &L, Visited &V, int incoming) {
     __threadfence();
     V.use_bits = true;
     V.dirty = true;
+    if (lane == 0 && V.stats) atomicAdd(&V.stats[0], 1);
 }
 
 // True if `id` was not yet visited in this layer; marks it.
@@ -683,11 +703,91 @@ HM_DEV uint32_t readlane_u32(uint32_t v, int j) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, j);
 }
 
+// Per-expansion features of the beam state, for the learned stop and its traces (float math).
+HM_DEV float slog(float x) { return x < 0.f ? -__log2f(1.f - x) : __log2f(1.f + x); }
+
 template <int MODE>
+HM_DEV float fdist(uint64_t k, int metric) {
+    if constexpr (MODE == kExactU8) {
+        if (metric == kIP) return (float)(int32_t)((uint32_t)k ^ 0x80000000u);
+        return (float)(uint32_t)k;
+    } else {
+        return (float)double_of_key(k, metric);
+    }
+}
+
+struct StopState {
+    int t = 0, dc0 = 0, since = 0, last_new = 0, last_acc = 0;
+    uint64_t last_kth = kInfKey;
+    float entry_slog = 0.f;
+};
+
+// The key of the candidate the search would expand next (the first unexpanded entry of R, else
+// the head of the tie list); the worst key of R when none is left.
+
+struct StopModel {
+    float w1[kFeatures], b1, w2;  // this lane's hidden unit
+    float mean[kFeatures], istd[kFeatures], b2;
+};
+
+HM_DEV void load_model(const SearchArgs &a, StopModel &M) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int i = 0; i < kFeatures; ++i) {
+        M.w1[i] = a.stop_w[kModelW1 + i * kHidden + lane];
+        M.mean[i] = a.stop_w[i];
+        M.istd[i] = a.stop_w[kFeatures + i];
+    }
+    M.b1 = a.stop_w[kModelB1 + lane];
+    M.w2 = a.stop_w[kModelW2 + lane];
+    M.b2 = a.stop_w[kModelB2];
+}
+
+// 16 -> 64 (ReLU) -> 1 with float32 accumulation; lane j owns hidden unit j.
+HM_DEV float model_logit(const StopModel &M, const float *f) {
+    float z = M.b1;
+#pragma unroll
+    for (int i = 0; i < kFeatures; ++i) z += M.w1[i] * ((f[i] - M.mean[i]) * M.istd[i]);
+    float y = M.w2 * fmaxf(z, 0.f);
+    for (int m = 32; m >= 1; m >>= 1) y += __shfl_xor(y, m, 64);
+    return y + M.b2;
+}
+
+template <int MODE>
+HM_DEV void features(const SearchArgs &a, const Query &Q, const StopState &S, uint64_t rk,
+                     int rn, uint64_t live, uint64_t next_key, float *f) {
+    const int kk = max(1, min(Q.k, rn));
+    const float dbest = fdist<MODE>(readlane_u64(rk, 0), a.metric);
+    const float dk = fdist<MODE>(readlane_u64(rk, kk - 1), a.metric);
+    const float dw = fdist<MODE>(readlane_u64(rk, rn - 1), a.metric);
+    const float dn = next_key == kInfKey ? dw : fdist<MODE>(next_key, a.metric);
+    const float span = fabsf(dk - dbest) + 1.f;
+    f[0] = (float)S.t;
+    f[1] = __log2f(1.f + (float)S.dc0);
+    f[2] = (float)Q.ef;
+    f[3] = (float)Q.k;
+    f[4] = (float)rn / (float)Q.ef;
+    f[5] = (float)popc(live) / (float)Q.ef;
+    f[6] = slog(dbest);
+    f[7] = slog(dk);
+    f[8] = slog(dn);
+    f[9] = (dn - dk) / span;
+    f[10] = (dk - dbest) / (fabsf(dbest) + 1.f);
+    f[11] = (dw - dk) / span;
+    f[12] = (float)S.since;
+    f[13] = (float)S.last_acc;
+    f[14] = (float)S.last_new;
+    f[15] = S.entry_slog - slog(dbest);
+}
+
+// AUX: 0 plain, 1 trace (plain search that records features and hits), 2 learned stop.
+template <int MODE, int AUX>
 HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                         uint64_t entry_k, uint32_t entry_id, uint64_t &rk, uint32_t &ri,
-                        int &rn) {
+                        int &rn, int qi, const StopModel &M) {
     const int lane = lane_id();
+    StopState S;
+    if constexpr (AUX != 0) S.entry_slog = slog(fdist<MODE>(entry_k, a.metric));
     const int ef = Q.ef;
     constexpr int kTies = 64;
     uint64_t *const tk = L.ck[0];  // evicted ties, sorted by id, [th, tn)
@@ -705,17 +805,24 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
         const uint64_t valid = rn >= 64 ? ~0ull : ((1ull << rn) - 1ull);
         const uint64_t live = valid & ~expd;
         uint32_t cid;
+        uint64_t ck;
         if (live) {
             const int p = __ffsll((long long)live) - 1;
             cid = readlane_u32(ri, p);
+            ck = readlane_u64(rk, p);
             expd |= 1ull << p;
         } else if (th < tn) {
             cid = ti[th];
+            ck = tk[th];
             th += 1;
         } else {
             break;
         }
         Q.hops += 1;
+        if constexpr (AUX != 0) {
+            S.last_new = 0;
+            S.last_acc = 0;
+        }
         for (int base = 0; base < a.w0; base += 64) {
             const int j = base + lane;
             const int32_t nb = j < a.w0 ? a.links0[(size_t)cid * a.w0 + j] : -1;
@@ -730,6 +837,10 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
             const uint64_t ki = distances<MODE>(a, L, nnew, Q.qq);
             const uint32_t ii = lane < nnew ? L.nbi[lane] : kInfId;
             Q.dcount += nnew;
+            if constexpr (AUX != 0) {
+                S.dc0 += nnew;
+                S.last_new += nnew;
+            }
             // accept_i <=> #{r in R: d_r <= d_i} + #{j < i new: d_j <= d_i} < ef
             int c_r = 0, c_n = 0;
             for (int q = 0; q < nnew; ++q) {
@@ -742,6 +853,7 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
             const uint64_t am = ballot(acc);
             if (am == 0) continue;
             const int na = popc(am);
+            if constexpr (AUX != 0) S.last_acc += na;
             // Merge positions: R entry -> lane + #{accepted below it}; accepted entry ->
             // #{R below it} + #{accepted below it}.
             int pos_r = lane, r_below = 0, a_below = 0;
@@ -788,7 +900,10 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                     const int ne = popc(em);
                     // compact the live ties to the front, then append the new ones and sort
                     const int live_t = tn - th;
-                    if (live_t + ne > kTies) return false;
+                    if (live_t + ne > kTies) {
+                        if (lane == 0 && a.stats) atomicAdd(&a.stats[1], 1);
+                        return false;
+                    }
                     uint64_t k0 = 0;
                     uint32_t i0 = 0;
                     if (lane < live_t) {  // live_t <= kTies = 64: one entry per lane
@@ -819,14 +934,52 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
             expd = ballot(nflag);
             rn = nrn;
         }
+        if constexpr (AUX != 0) {
+            S.t += 1;
+            const int kk = max(1, min(Q.k, rn));
+            const uint64_t kth = readlane_u64(rk, kk - 1);
+            S.since = kth == S.last_kth ? S.since + 1 : 0;
+            S.last_kth = kth;
+            const uint64_t valid_now = rn >= 64 ? ~0ull : ((1ull << rn) - 1ull);
+            const uint64_t live_now = valid_now & ~expd;
+            uint64_t next_key = kInfKey;
+            if (live_now) next_key = readlane_u64(rk, __ffsll((long long)live_now) - 1);
+            else if (th < tn) next_key = tk[th];
+            float f[kFeatures];
+            features<MODE>(a, Q, S, rk, rn, live_now, next_key, f);
+            if constexpr (AUX == 1) {
+                const double thr = a.trace_thr[qi];
+                const int hits = popc(ballot(lane < kk && decode<MODE>(rk, a.metric) <= thr));
+                (void)ck;
+                if (S.t <= a.trace_steps) {
+                    float *row = a.trace + ((size_t)qi * a.trace_steps + (S.t - 1)) * kTraceCols;
+                    if (lane < kFeatures) {
+                        float v = 0.f;
+#pragma unroll
+                        for (int i = 0; i < kFeatures; ++i)
+                            if (lane == i) v = f[i];
+                        row[lane] = v;
+                    }
+                    if (lane == kFeatures) row[lane] = (float)hits;
+                    if (lane == kFeatures + 1) row[lane] = (float)rn;
+                }
+                if (lane == 0) {
+                    a.trace_len[qi] = S.t;
+                    a.trace_final[qi] = hits;
+                }
+            } else {
+                if (S.t >= a.stop_tmin && model_logit(M, f) > a.stop_logit) break;
+            }
+        }
     }
     return true;
 }
 
 // ---------------------------------------------------------------------------- one query
 
-template <int MODE, bool FILT, bool FAST>
-HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int slot) {
+template <int MODE, bool FILT, bool FAST, int AUX>
+HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int slot,
+                      const StopModel &M) {
     const int lane = lane_id();
     Query Q;
     Q.ef = a.ef_q ? a.ef_q[qi] : a.ef_all;
@@ -874,7 +1027,7 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
     if constexpr (FAST) {
         uint64_t rk;
         uint32_t ri;
-        ok = layer0_fast<MODE>(a, L, V, Q, cur_k, cur_id, rk, ri, rn);
+        ok = layer0_fast<MODE, AUX>(a, L, V, Q, cur_k, cur_id, rk, ri, rn, qi, M);
         if (!ok) rn = 0;
         const int take = min(Q.k, rn);
         for (int i = lane; i < a.kstride; i += 64) {
@@ -905,23 +1058,26 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
     wave_sync();
 }
 
-template <int MODE, bool FILT, bool FAST>
+template <int MODE, bool FILT, bool FAST, int AUX>
 __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
     extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
     const int slot = blockIdx.x;
-    const Lds L = carve(lds_raw, a, MODE);
+    const Lds L = FAST ? carve_fast(lds_raw, a, MODE) : carve(lds_raw, a, MODE);
     Visited V;
     V.bits = a.bitsets + (size_t)slot * a.bitset_words;
+    V.stats = a.stats;
     V.dirty = false;
     V.use_bits = false;
     V.hlog = a.h_log2;
     V.count = 0;
+    StopModel M;
+    if constexpr (AUX == 2) load_model(a, M);
     for (;;) {
         int qi = 0;
         if (lane_id() == 0) qi = atomicAdd(a.queue, 1);
         qi = uniform(shfl(qi, 0));
         if (qi >= a.nq) break;
-        run_query<MODE, FILT, FAST>(a, L, V, qi, slot);
+        run_query<MODE, FILT, FAST, AUX>(a, L, V, qi, slot, M);
     }
     // Leave the bitset clean for the next launch.
     if (V.dirty) {
@@ -931,14 +1087,18 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
 
 __global__ void reset_queue_kernel(int32_t *q) { q[0] = 0; }
 
-template __global__ void search_kernel<kExactU8, false, false>(SearchArgs);
-template __global__ void search_kernel<kExactU8, true, false>(SearchArgs);
-template __global__ void search_kernel<kSeqF32, false, false>(SearchArgs);
-template __global__ void search_kernel<kSeqF32, true, false>(SearchArgs);
-template __global__ void search_kernel<kSeqU8, false, false>(SearchArgs);
-template __global__ void search_kernel<kExactU8, false, true>(SearchArgs);
-template __global__ void search_kernel<kSeqF32, false, true>(SearchArgs);
-template __global__ void search_kernel<kSeqU8, false, true>(SearchArgs);
+template __global__ void search_kernel<kExactU8, false, false, 0>(SearchArgs);
+template __global__ void search_kernel<kExactU8, true, false, 0>(SearchArgs);
+template __global__ void search_kernel<kSeqF32, false, false, 0>(SearchArgs);
+template __global__ void search_kernel<kSeqF32, true, false, 0>(SearchArgs);
+template __global__ void search_kernel<kSeqU8, false, false, 0>(SearchArgs);
+template __global__ void search_kernel<kExactU8, false, true, 0>(SearchArgs);
+template __global__ void search_kernel<kSeqF32, false, true, 0>(SearchArgs);
+template __global__ void search_kernel<kSeqU8, false, true, 0>(SearchArgs);
+template __global__ void search_kernel<kExactU8, false, true, 1>(SearchArgs);
+template __global__ void search_kernel<kSeqF32, false, true, 1>(SearchArgs);
+template __global__ void search_kernel<kExactU8, false, true, 2>(SearchArgs);
+template __global__ void search_kernel<kSeqF32, false, true, 2>(SearchArgs);
 
 }  // namespace hm
 
@@ -948,22 +1108,28 @@ namespace hm {
 
 using KernelFn = void (*)(SearchArgs);
 
-static KernelFn pick(int mode, bool filt, bool fast) {
+static KernelFn pick(int mode, bool filt, bool fast, int aux) {
+    if (aux == 1) return mode == kExactU8 ? search_kernel<kExactU8, false, true, 1>
+                                          : search_kernel<kSeqF32, false, true, 1>;
+    if (aux == 2) return mode == kExactU8 ? search_kernel<kExactU8, false, true, 2>
+                                          : search_kernel<kSeqF32, false, true, 2>;
     if (fast && !filt) {
-        if (mode == kExactU8) return search_kernel<kExactU8, false, true>;
-        if (mode == kSeqF32) return search_kernel<kSeqF32, false, true>;
-        return search_kernel<kSeqU8, false, true>;
+        if (mode == kExactU8) return search_kernel<kExactU8, false, true, 0>;
+        if (mode == kSeqF32) return search_kernel<kSeqF32, false, true, 0>;
+        return search_kernel<kSeqU8, false, true, 0>;
     }
     if (mode == kExactU8)
-        return filt ? search_kernel<kExactU8, true, false> : search_kernel<kExactU8, false, false>;
+        return filt ? search_kernel<kExactU8, true, false, 0> : search_kernel<kExactU8, false, false, 0>;
     if (mode == kSeqF32)
-        return filt ? search_kernel<kSeqF32, true, false> : search_kernel<kSeqF32, false, false>;
-    return search_kernel<kSeqU8, false, false>;
+        return filt ? search_kernel<kSeqF32, true, false, 0> : search_kernel<kSeqF32, false, false, 0>;
+    return search_kernel<kSeqU8, false, false, 0>;
 }
 
+static int aux_of(const SearchArgs &a) { return a.trace ? 1 : (a.stop_w ? 2 : 0); }
+
 // Blocks (one wave each) that fit on the device at once for this LDS size.
-int search_slots(int mode, bool filt, bool fast, int lds, int device_cus) {
-    KernelFn fn = pick(mode, filt, fast);
+int search_slots(int mode, bool filt, bool fast, int aux, int lds, int device_cus) {
+    KernelFn fn = pick(mode, filt, fast, aux);
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     int per_cu = 0;
@@ -975,7 +1141,7 @@ int search_slots(int mode, bool filt, bool fast, int lds, int device_cus) {
 }
 
 hipError_t search_launch(int mode, bool filt, const SearchArgs &a, int blocks, hipStream_t s) {
-    KernelFn fn = pick(mode, filt, a.fast != 0);
+    KernelFn fn = pick(mode, filt, a.fast != 0, aux_of(a));
     if (a.lds_per_wave > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)fn,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -59,7 +59,42 @@ struct SearchArgs {
     int lds_per_wave;
     int slots;
     int fast;  // 1: unfiltered, every ef <= 64 -> the register-beam layer-0 kernel
+    // learned stopping (register-beam kernel only): model parameters (layout below) and the
+    // logit above which a beam stops; at least stop_tmin layer-0 expansions run first
+    const float *stop_w;
+    float stop_logit;
+    int stop_tmin;
+    // traces of the plain register-beam search: per query up to trace_steps rows of
+    // kTraceCols floats, the number of expansions and the final tie-aware hit count
+    float *trace;
+    int trace_steps;
+    const double *trace_thr;  // per query: exact k-th distance plus the recall tolerance
+    int32_t *trace_len;
+    int32_t *trace_final;
+    // event counters (atomic adds): [0] layer searches whose visited set moved from the LDS
+    // hash to the global bitset, [1] queries whose evicted ties outgrew the register beam
+    int32_t *stats;
 };
+
+// Learned-stop model parameters (float32, weights already rounded to bfloat16):
+// [0,16) feature means, [16,32) inverse standard deviations, [32, 32+16*64) W1 (row i: feature
+// i's weights to the 64 hidden units), then b1 (64), W2 (64), b2 (1).
+constexpr int kFeatures = 16;
+constexpr int kHidden = 64;
+constexpr int kModelW1 = 32;
+constexpr int kModelB1 = kModelW1 + kFeatures * kHidden;
+constexpr int kModelW2 = kModelB1 + kHidden;
+constexpr int kModelB2 = kModelW2 + kHidden;
+constexpr int kModelSize = kModelB2 + 1;
+constexpr int kTraceCols = kFeatures + 2;  // features, hits, |R|
+
+// Bytes of LDS one wave of the register-beam kernel needs: query, 64-entry R scatter buffer,
+// flags, 64-entry tie list, the new-neighbour staging and the visited hash.
+inline int lds_bytes_fast(int mode, int dim, int dimp, int h_log2) {
+    auto up = [](int v) { return (v + 15) & ~15; };
+    int q = mode == kExactU8 ? up(dimp) : up(dim * 8);
+    return q + 2 * (64 * 8 + 64 * 4) + 64 * 4 + 64 * 4 + 64 * 4 + (4 << h_log2);
+}
 
 // Bytes of LDS one wave needs for the given capacities.
 inline int lds_bytes_per_wave(int mode, int dim, int dimp, int r_cap, int c_cap, int h_log2) {

@@ -26,6 +26,15 @@ SYMBOLS = (
     "hm_rerank",
     "hm_exact",
     "hm_build",
+    "hm_model_size",
+    "hm_set_stop_model",
+    "hm_trace_dev",
+    "hm_trace",
+    "hm_mlp_grad",
+    "hm_mlp_forward",
+    "hm_mlp_train",
+    "hm_set_tuning",
+    "hm_read_stats",
     "hm_last_error",
 )
 
@@ -64,6 +73,7 @@ class SearchOpts(ctypes.Structure):
         ("n_masks", ctypes.c_int64),
         ("mask_index", ctypes.c_void_p),
         ("mask_all", ctypes.c_int32),
+        ("learned", ctypes.c_int32),
     ]
 
 
@@ -103,6 +113,16 @@ def lib() -> ctypes.CDLL:
                 [i64, i64, i32, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp],
                 ctypes.c_int,
             ),
+            "hm_model_size": ([], ctypes.c_int),
+            "hm_set_stop_model": ([vp, vp, ctypes.c_float, i32], ctypes.c_int),
+            "hm_trace_dev": ([vp, vp, i64, i32, i32, ctypes.c_int, vp, vp, i32, vp, vp, vp], ctypes.c_int),
+            "hm_trace": ([vp, vp, i64, i32, i32, vp, vp, i32, vp, vp], ctypes.c_int),
+            "hm_mlp_grad": ([vp, vp, i64, vp, ctypes.c_float, vp, vp], ctypes.c_int),
+            "hm_mlp_forward": ([vp, i64, vp, vp], ctypes.c_int),
+            "hm_mlp_train": ([vp, vp, i64, vp, i32, i32, ctypes.c_float, ctypes.c_float,
+                              ctypes.c_uint64, vp], ctypes.c_int),
+            "hm_set_tuning": ([vp, i32, i32, i32], ctypes.c_int),
+            "hm_read_stats": ([vp, vp, ctypes.c_int], ctypes.c_int),
             "hm_last_error": ([], ctypes.c_char_p),
         }
         for name, (args, res) in sig.items():

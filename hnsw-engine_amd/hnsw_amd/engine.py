@@ -149,6 +149,7 @@ class MI355XEngine:
         mask_pool: np.ndarray | None = None,
         mask_index=None,
         kstride: int | None = None,
+        learned: bool = False,
     ):
         """Batch search with per-query ``k``/``ef`` (ints or arrays) and per-query masks
         (``mask_pool`` rows, ``mask_index`` per query or one int; -1: unfiltered). Returns
@@ -157,6 +158,7 @@ class MI355XEngine:
         q = np.ascontiguousarray(queries, dtype=np.float32)
         nq = q.shape[0]
         opts, keep, kmax = self._opts(nq, k, ef, mask_pool, mask_index, kstride)
+        opts.learned = int(bool(learned))
         ids = np.empty((nq, kmax), dtype=np.int64)
         dists = np.empty((nq, kmax), dtype=np.float64)
         dc = np.zeros(nq, dtype=np.int64)
@@ -215,6 +217,33 @@ class MI355XEngine:
                 )
             )
         return ids, dists
+
+    def set_stop_model(self, params: np.ndarray | None, threshold: float = 0.5,
+                       min_steps: int = 0) -> None:
+        """Install (or remove, with ``None``) the learned-stop model used by ``learned=True``
+        searches."""
+        handle = self._device()
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        _native.check(_native.lib().hm_set_stop_model(handle, _native.ptr(p), float(threshold),
+                                                      int(min_steps)))
+        self._stop = (p, float(threshold), int(min_steps))
+
+    def trace(self, queries: np.ndarray, k: int, ef: int, thresholds: np.ndarray, steps: int = 128):
+        """Plain register-beam search recording the stop features per layer-0 expansion.
+
+        Returns ``(trace (nq, steps, 18), length (nq,), final_hits (nq,))``; ``thresholds`` is
+        each query's exact k-th distance plus the recall tolerance.
+        """
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = q.shape[0]
+        thr = np.ascontiguousarray(thresholds, dtype=np.float64)
+        tr = np.zeros((nq, steps, 18), dtype=np.float32)
+        ln = np.zeros(nq, dtype=np.int32)
+        fin = np.zeros(nq, dtype=np.int32)
+        _native.check(_native.lib().hm_trace(
+            self._device(), _native.ptr(q), nq, int(k), int(ef), _native.ptr(thr),
+            _native.ptr(tr), int(steps), _native.ptr(ln), _native.ptr(fin)))
+        return tr, ln, fin
 
     # -- state ------------------------------------------------------------------------------
 
@@ -317,6 +346,10 @@ class MI355XEngine:
             _native.check(_native.lib().hm_open(ctypes.byref(g), self.device, ctypes.byref(handle)))
             self._handle = handle
             self._codes = None
+            stop = getattr(self, "_stop", None)
+            if stop is not None and stop[0] is not None:
+                _native.check(_native.lib().hm_set_stop_model(
+                    handle, _native.ptr(stop[0]), stop[1], stop[2]))
         return self._handle
 
     def _invalidate(self) -> None:

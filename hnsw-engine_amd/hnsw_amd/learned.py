@@ -1,0 +1,108 @@
+"""Learned early stopping of the layer-0 beam (no counterpart in the reference).
+
+After every layer-0 expansion the register-beam kernel evaluates a 16 -> 64 -> 1 MLP
+(bfloat16 weights, float32 accumulation) on features of the beam state and stops when the
+predicted probability that the current top-k is already final exceeds a threshold. The plain
+fixed-ef search is the fallback: a model that never fires reproduces it exactly, and the stop can
+only shorten a search.
+
+Pipeline (all on the GPU): traces of the plain search on training queries (features and the
+tie-aware hits of the current top-k against the exact oracle, per expansion), labels
+``hits_t == hits_final`` (stopping here loses nothing against running the full beam), training
+by the library's own forward/backward and Adam kernels, then a threshold chosen on held-out
+queries for the recall target.
+
+Feature order (``features`` in search.hip): expansions so far, log2(1 + distances computed),
+ef, k, |R|/ef, unexpanded entries/ef, signed log2 of the best / k-th / worst / popped distance,
+(popped - best)/(|k-th - best| + 1), (k-th - best)/(|best| + 1), expansions since the k-th
+changed, accepted and new neighbours of the last expansion, log-ratio of the layer-0 entry
+distance to the best.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+from .recall import RELATIVE_TIE_TOLERANCE
+
+N_FEATURES = 16
+HIDDEN = 64
+MODEL_W1 = 32
+MODEL_B1 = MODEL_W1 + N_FEATURES * HIDDEN
+MODEL_W2 = MODEL_B1 + HIDDEN
+MODEL_B2 = MODEL_W2 + HIDDEN
+MODEL_SIZE = MODEL_B2 + 1
+
+
+def recall_thresholds(true_dist: np.ndarray, k: int) -> np.ndarray:
+    """Per query: exact k-th distance plus the tie tolerance of the recall definition."""
+    kth = true_dist[:, k - 1]
+    return kth + RELATIVE_TIE_TOLERANCE * np.maximum(1.0, np.abs(kth))
+
+
+def dataset_from_traces(trace: np.ndarray, length: np.ndarray, final: np.ndarray,
+                        max_rows: int | None = None, seed: int = 0):
+    """``(x, y)``: one row per recorded expansion; label 1 when the hits already equal the
+    final hits. With ``max_rows`` a uniform sample (fixed seed) of that many rows."""
+    steps = trace.shape[1]
+    t_idx = np.arange(steps)[None, :]
+    valid = t_idx < np.minimum(length, steps)[:, None]
+    x = trace[..., :N_FEATURES][valid]
+    hits = trace[..., N_FEATURES][valid]
+    fin = np.broadcast_to(final[:, None], valid.shape)[valid]
+    y = (hits >= fin).astype(np.float32)
+    if max_rows is not None and x.shape[0] > max_rows:
+        pick = np.sort(np.random.default_rng(seed).choice(x.shape[0], max_rows, replace=False))
+        x, y = x[pick], y[pick]
+    return np.ascontiguousarray(x, dtype=np.float32), np.ascontiguousarray(y, dtype=np.float32)
+
+
+def init_model(x: np.ndarray, seed: int = 0) -> np.ndarray:
+    """Feature normalisation from ``x`` and He-initialised weights."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(MODEL_SIZE, dtype=np.float32)
+    mean = x.mean(axis=0)
+    std = x.std(axis=0)
+    p[:N_FEATURES] = mean
+    p[N_FEATURES:MODEL_W1] = 1.0 / np.where(std > 1e-6, std, 1.0)
+    p[MODEL_W1:MODEL_B1] = (rng.standard_normal(N_FEATURES * HIDDEN) * np.sqrt(2.0 / N_FEATURES))
+    p[MODEL_W2:MODEL_B2] = rng.standard_normal(HIDDEN) * np.sqrt(1.0 / HIDDEN)
+    return p
+
+
+def train(x: np.ndarray, y: np.ndarray, *, epochs: int = 12, batch: int = 1024,
+          lr: float = 3e-3, pos_weight: float = 1.0, seed: int = 0):
+    """GPU training (``hm_mlp_train``); returns ``(params, epoch_losses)``."""
+    params = init_model(x, seed)
+    losses = np.zeros(epochs, dtype=np.float32)
+    xc = np.ascontiguousarray(x, dtype=np.float32)
+    yc = np.ascontiguousarray(y, dtype=np.float32)
+    _native.check(_native.lib().hm_mlp_train(
+        _native.ptr(xc), _native.ptr(yc), xc.shape[0], _native.ptr(params), epochs, batch,
+        ctypes.c_float(lr), ctypes.c_float(pos_weight), ctypes.c_uint64(seed), _native.ptr(losses)))
+    return params, losses
+
+
+def gradient(x: np.ndarray, y: np.ndarray, params: np.ndarray, pos_weight: float = 1.0):
+    """``(grad, loss)`` of the mean loss over ``x`` (``hm_mlp_grad``)."""
+    xc = np.ascontiguousarray(x, dtype=np.float32)
+    yc = np.ascontiguousarray(y, dtype=np.float32)
+    pc = np.ascontiguousarray(params, dtype=np.float32)
+    g = np.zeros(MODEL_SIZE, dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    _native.check(_native.lib().hm_mlp_grad(_native.ptr(xc), _native.ptr(yc), xc.shape[0],
+                                            _native.ptr(pc), ctypes.c_float(pos_weight),
+                                            _native.ptr(g), _native.ptr(loss)))
+    return g, float(loss[0])
+
+
+def logits(x: np.ndarray, params: np.ndarray) -> np.ndarray:
+    xc = np.ascontiguousarray(x, dtype=np.float32)
+    pc = np.ascontiguousarray(params, dtype=np.float32)
+    out = np.zeros(xc.shape[0], dtype=np.float32)
+    _native.check(_native.lib().hm_mlp_forward(_native.ptr(xc), xc.shape[0], _native.ptr(pc),
+                                               _native.ptr(out)))
+    return out

@@ -32,6 +32,17 @@ def clustered_u8(
     return data, queries
 
 
+def clustered_u8_queries(
+    n_queries: int, dim: int, *, n_clusters: int, sigma: float = 30.0, seed: int = 0,
+    query_seed: int,
+) -> np.ndarray:
+    """More queries around the centres of ``clustered_u8(..., seed=seed)`` (the centres are the
+    first draw of that generator), from ``query_seed`` -- e.g. training queries disjoint from
+    the measured ones."""
+    centres = np.random.default_rng(seed).uniform(0.0, 255.0, size=(n_clusters, dim))
+    return _around(np.random.default_rng(query_seed), centres, n_queries, sigma)
+
+
 def _around(rng: np.random.Generator, centres: np.ndarray, n: int, sigma: float) -> np.ndarray:
     out = np.empty((n, centres.shape[1]), dtype=np.float32)
     labels = rng.integers(0, centres.shape[0], size=n)

@@ -23,7 +23,7 @@ from pathlib import Path
 import numpy as np
 
 from .engine import MI355XEngine
-from .synthetic import clustered_u8
+from .synthetic import clustered_u8, clustered_u8_queries
 
 CACHE = Path(__file__).resolve().parents[2] / ".graph_cache"
 
@@ -45,6 +45,11 @@ class Workload:
     def data(self) -> tuple[np.ndarray, np.ndarray]:
         return clustered_u8(self.n, self.dim, self.n_queries, n_clusters=self.n_clusters,
                             sigma=self.sigma, seed=self.seed)
+
+    def train_queries(self, count: int) -> np.ndarray:
+        """Queries from the same clusters, disjoint from the measured ones (seed + 2)."""
+        return clustered_u8_queries(count, self.dim, n_clusters=self.n_clusters,
+                                    sigma=self.sigma, seed=self.seed, query_seed=self.seed + 2)
 
     def levels(self) -> np.ndarray:
         rng = np.random.default_rng(self.level_seed)

@@ -65,7 +65,9 @@ typedef struct hm_index hm_index;
  * does (numba_engine.py:473,487). Masks: a pool of n_masks boolean rows of n bytes
  * (mask_pool, n_masks x n); mask_index gives each query its row (-1: unfiltered) or, when
  * null, mask_all applies to every query (-1: unfiltered). Outputs are (nq, kstride) with
- * -1 / +inf padding; kstride >= max k. */
+ * -1 / +inf padding; kstride >= max k. learned = 1 applies the stop model of
+ * hm_set_stop_model to unfiltered batches with ef <= 64 (a stopped beam returns its current
+ * top-k; every other batch, or a model that never fires, is the plain fixed-ef search). */
 typedef struct {
     int64_t nq;
     const int32_t *k;
@@ -77,6 +79,7 @@ typedef struct {
     int64_t n_masks;
     const int32_t *mask_index;
     int32_t mask_all;
+    int32_t learned;
 } hm_search_opts;
 
 /* Uploads a validated graph to HBM (validation is the caller's: the kernels do not
@@ -141,6 +144,51 @@ int hm_build(int64_t start, int64_t n, int32_t dim, int32_t metric, int32_t M,
              int32_t keep_pruned, const float *vectors, const int64_t *levels, int32_t *links0,
              int32_t *count0, const int64_t *upper_offset, int32_t *links_up, int32_t *count_up,
              int64_t *state, int64_t *dcount);
+
+/* Learned stop (no counterpart in the reference; an extension that only ever stops a beam
+ * earlier than the plain fixed-ef search). A model is hm_model_size() floats: feature means
+ * (16), inverse standard deviations (16), W1 (16 x 64, row i = feature i), b1 (64), W2 (64), b2;
+ * weights are used rounded to bfloat16, accumulation is float32. */
+int hm_model_size(void);
+
+/* Installs (params) or removes (null) the stop model: a beam stops after an expansion when the
+ * model's probability that the current top-k is final exceeds threshold, once at least
+ * min_steps layer-0 expansions ran. */
+int hm_set_stop_model(hm_index *index, const float *params, float threshold, int32_t min_steps);
+
+/* Plain register-beam search (unfiltered, k <= ef <= 64, device buffers) that records, for
+ * each query and each of its first `steps` layer-0 expansions, the 16 stop features, the number
+ * of its current top-k whose distance is <= d_thr[query] (the exact k-th distance plus the
+ * recall tolerance) and |R|: d_trace is (nq, steps, 18) float32. d_len: expansions per query;
+ * d_final: hits of the final result. */
+int hm_trace_dev(hm_index *index, const float *d_queries, int64_t nq, int32_t k, int32_t ef,
+                 int exact_u8, const double *d_thr, float *d_trace, int32_t steps,
+                 int32_t *d_len, int32_t *d_final, void *stream);
+
+/* hm_trace_dev with host buffers (the library allocates and frees the device copies). */
+int hm_trace(hm_index *index, const float *queries, int64_t nq, int32_t k, int32_t ef,
+             const double *thr, float *trace, int32_t steps, int32_t *len, int32_t *final_hits);
+
+/* Training kernels (host buffers; x: (n, 16) raw features, y: (n,) 0/1 labels):
+ * hm_mlp_grad -- mean binary cross-entropy gradient (positives weighted by pos_weight) of a
+ * model, and the loss; hm_mlp_forward -- logits; hm_mlp_train -- Adam over shuffled mini-batches
+ * (deterministic for a seed), params in/out, epoch_loss (epochs floats) or null. */
+int hm_mlp_grad(const float *x, const float *y, int64_t n, const float *params, float pos_weight,
+                float *grad, float *loss);
+int hm_mlp_forward(const float *x, int64_t n, const float *params, float *logits);
+int hm_mlp_train(const float *x, const float *y, int64_t n, float *params, int32_t epochs,
+                 int32_t batch, float lr, float pos_weight, uint64_t seed, float *epoch_loss);
+
+/* Tuning knobs (0 restores the automatic choice): log2 of the visited-hash slots per wave of the
+ * register-beam kernel and of the general kernel (larger hashes spill to the global bitset
+ * later but leave fewer waves per compute unit), and a cap on resident waves per compute unit. */
+int hm_set_tuning(hm_index *index, int32_t fast_hash_log2, int32_t hash_log2,
+                  int32_t waves_per_cu);
+
+/* Event counters since the last reset: out[0] layer searches whose visited set outgrew the LDS
+ * hash, out[1] queries whose evicted ties outgrew the register beam (re-run on the general
+ * kernel by hm_search). Synchronises the device. */
+int hm_read_stats(hm_index *index, int32_t *out, int reset);
 
 const char *hm_last_error(void);
 

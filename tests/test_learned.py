@@ -1,0 +1,49 @@
+"""CPU checks of the learned-stop pieces: the float64 MLP reference and the trace labelling."""
+
+from __future__ import annotations
+
+import numpy as np
+
+import oracle.mlp as mlp
+from hnsw_amd.learned import MODEL_SIZE, dataset_from_traces, init_model
+
+
+def test_reference_gradient_matches_finite_differences() -> None:
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((64, 16)) * 3 + 1
+    y = (rng.random(64) < 0.4).astype(np.float64)
+    p = init_model(x.astype(np.float32), seed=1).astype(np.float64)
+    p[mlp.B1:mlp.W2] = rng.standard_normal(64) * 0.1
+    loss, g = mlp.loss_and_grad(x, y, p, pos_weight=2.0, round_weights=False)
+    eps = 1e-6
+    for i in list(range(mlp.W1, mlp.W1 + 40)) + list(range(mlp.B1, mlp.SIZE)):
+        q = p.copy()
+        q[i] += eps
+        lp, _ = mlp.loss_and_grad(x, y, q, pos_weight=2.0, round_weights=False)
+        q[i] -= 2 * eps
+        lm, _ = mlp.loss_and_grad(x, y, q, pos_weight=2.0, round_weights=False)
+        assert abs((lp - lm) / (2 * eps) - g[i]) < 1e-6 * max(1.0, abs(g[i])), i
+    assert g[:mlp.W1].sum() == 0.0
+    assert mlp.SIZE == MODEL_SIZE
+
+
+def test_bf16_rounding_is_round_to_nearest_even() -> None:
+    vals = np.array([1.0, 1.0 + 2**-8, 1.0 + 3 * 2**-9, -2.5, 1.0 + 2**-9], dtype=np.float32)
+    got = mlp.bf16(vals)
+    # 1 + 2^-8 is halfway between 1 and 1 + 2^-7 (bfloat16 keeps 7 fraction bits): ties go to
+    # the even neighbour 1.0; 1 + 3*2^-9 rounds up; 1 + 2^-9 rounds down.
+    np.testing.assert_array_equal(got, np.array([1.0, 1.0, 1.0 + 2**-7, -2.5, 1.0],
+                                                dtype=np.float32))
+
+
+def test_labels_mark_steps_whose_hits_equal_the_final_hits() -> None:
+    trace = np.zeros((2, 5, 18), dtype=np.float32)
+    trace[0, :, 16] = [1, 3, 3, 5, 9]
+    trace[1, :, 16] = [0, 2, 2, 0, 0]
+    trace[..., 0] = np.arange(5)
+    length = np.array([5, 3], dtype=np.int32)
+    final = np.array([5, 2], dtype=np.int32)
+    x, y = dataset_from_traces(trace, length, final)
+    assert x.shape == (8, 16)
+    np.testing.assert_array_equal(y, [0, 0, 0, 1, 1, 0, 1, 1])
+    np.testing.assert_array_equal(x[:, 0], [0, 1, 2, 3, 4, 0, 1, 2])
