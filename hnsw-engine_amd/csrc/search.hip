@@ -719,25 +719,21 @@ HM_DEV float fdist(uint64_t k, int metric) {
 struct StopState {
     int t = 0, dc0 = 0, since = 0, last_new = 0, last_acc = 0;
     uint64_t last_kth = kInfKey;
-    float entry_slog = 0.f;
 };
 
 // The key of the candidate the search would expand next (the first unexpanded entry of R, else
 // the head of the tie list); the worst key of R when none is left.
 
+// The model as the search kernel holds it: the feature normalisation is folded into the first
+// layer at upload (w1' = w1 / std, b1' = b1 - sum w1' mean), so a lane keeps 18 values.
 struct StopModel {
-    float w1[kFeatures], b1, w2;  // this lane's hidden unit
-    float mean[kFeatures], istd[kFeatures], b2;
+    float w1[kFeatures], b1, w2, b2;  // this lane's hidden unit
 };
 
 HM_DEV void load_model(const SearchArgs &a, StopModel &M) {
     const int lane = lane_id();
 #pragma unroll
-    for (int i = 0; i < kFeatures; ++i) {
-        M.w1[i] = a.stop_w[kModelW1 + i * kHidden + lane];
-        M.mean[i] = a.stop_w[i];
-        M.istd[i] = a.stop_w[kFeatures + i];
-    }
+    for (int i = 0; i < kFeatures; ++i) M.w1[i] = a.stop_w[kModelW1 + i * kHidden + lane];
     M.b1 = a.stop_w[kModelB1 + lane];
     M.w2 = a.stop_w[kModelW2 + lane];
     M.b2 = a.stop_w[kModelB2];
@@ -747,7 +743,7 @@ HM_DEV void load_model(const SearchArgs &a, StopModel &M) {
 HM_DEV float model_logit(const StopModel &M, const float *f) {
     float z = M.b1;
 #pragma unroll
-    for (int i = 0; i < kFeatures; ++i) z += M.w1[i] * ((f[i] - M.mean[i]) * M.istd[i]);
+    for (int i = 0; i < kFeatures; ++i) z += M.w1[i] * f[i];
     float y = M.w2 * fmaxf(z, 0.f);
     for (int m = 32; m >= 1; m >>= 1) y += __shfl_xor(y, m, 64);
     return y + M.b2;
@@ -761,23 +757,24 @@ HM_DEV void features(const SearchArgs &a, const Query &Q, const StopState &S, ui
     const float dk = fdist<MODE>(readlane_u64(rk, kk - 1), a.metric);
     const float dw = fdist<MODE>(readlane_u64(rk, rn - 1), a.metric);
     const float dn = next_key == kInfKey ? dw : fdist<MODE>(next_key, a.metric);
-    const float span = fabsf(dk - dbest) + 1.f;
+    const float inv_span = __frcp_rn(fabsf(dk - dbest) + 1.f);
+    const uint64_t topk = kk >= 64 ? ~0ull : ((1ull << kk) - 1ull);
     f[0] = (float)S.t;
     f[1] = __log2f(1.f + (float)S.dc0);
     f[2] = (float)Q.ef;
     f[3] = (float)Q.k;
-    f[4] = (float)rn / (float)Q.ef;
-    f[5] = (float)popc(live) / (float)Q.ef;
+    f[4] = (float)rn * __frcp_rn((float)Q.ef);
+    f[5] = (float)popc(live) * __frcp_rn((float)Q.ef);
     f[6] = slog(dbest);
     f[7] = slog(dk);
     f[8] = slog(dn);
-    f[9] = (dn - dk) / span;
-    f[10] = (dk - dbest) / (fabsf(dbest) + 1.f);
-    f[11] = (dw - dk) / span;
+    f[9] = (dn - dk) * inv_span;
+    f[10] = (dk - dbest) * __frcp_rn(fabsf(dbest) + 1.f);
+    f[11] = (dw - dk) * inv_span;
     f[12] = (float)S.since;
     f[13] = (float)S.last_acc;
     f[14] = (float)S.last_new;
-    f[15] = S.entry_slog - slog(dbest);
+    f[15] = (float)popc(live & topk);  // top-k entries not yet expanded
 }
 
 // AUX: 0 plain, 1 trace (plain search that records features and hits), 2 learned stop.
@@ -787,7 +784,6 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                         int &rn, int qi, const StopModel &M) {
     const int lane = lane_id();
     StopState S;
-    if constexpr (AUX != 0) S.entry_slog = slog(fdist<MODE>(entry_k, a.metric));
     const int ef = Q.ef;
     constexpr int kTies = 64;
     uint64_t *const tk = L.ck[0];  // evicted ties, sorted by id, [th, tn)
@@ -967,8 +963,9 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                     a.trace_len[qi] = S.t;
                     a.trace_final[qi] = hits;
                 }
-            } else {
-                if (S.t >= a.stop_tmin && model_logit(M, f) > a.stop_logit) break;
+            } else if (S.t >= a.stop_tmin && (S.t & 1) == 0) {
+                // every other expansion: the model is evaluated as often as it pays
+                if (model_logit(M, f) > a.stop_logit) break;
             }
         }
     }
@@ -1059,7 +1056,7 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
 }
 
 template <int MODE, bool FILT, bool FAST, int AUX>
-__global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
+HM_DEV void search_loop(const SearchArgs &a) {
     extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
     const int slot = blockIdx.x;
     const Lds L = FAST ? carve_fast(lds_raw, a, MODE) : carve(lds_raw, a, MODE);
@@ -1085,20 +1082,33 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
     }
 }
 
+// General kernel (any ef, filters): LDS-bound, so no occupancy target for the compiler.
+template <int MODE, bool FILT>
+__global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
+    search_loop<MODE, FILT, false, 0>(a);
+}
+
+// Register-beam kernel: four waves per SIMD (<= 128 VGPRs), as many as its LDS allows.
+template <int MODE, int AUX>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void search_fast_kernel(
+    SearchArgs a) {
+    search_loop<MODE, false, true, AUX>(a);
+}
+
 __global__ void reset_queue_kernel(int32_t *q) { q[0] = 0; }
 
-template __global__ void search_kernel<kExactU8, false, false, 0>(SearchArgs);
-template __global__ void search_kernel<kExactU8, true, false, 0>(SearchArgs);
-template __global__ void search_kernel<kSeqF32, false, false, 0>(SearchArgs);
-template __global__ void search_kernel<kSeqF32, true, false, 0>(SearchArgs);
-template __global__ void search_kernel<kSeqU8, false, false, 0>(SearchArgs);
-template __global__ void search_kernel<kExactU8, false, true, 0>(SearchArgs);
-template __global__ void search_kernel<kSeqF32, false, true, 0>(SearchArgs);
-template __global__ void search_kernel<kSeqU8, false, true, 0>(SearchArgs);
-template __global__ void search_kernel<kExactU8, false, true, 1>(SearchArgs);
-template __global__ void search_kernel<kSeqF32, false, true, 1>(SearchArgs);
-template __global__ void search_kernel<kExactU8, false, true, 2>(SearchArgs);
-template __global__ void search_kernel<kSeqF32, false, true, 2>(SearchArgs);
+template __global__ void search_kernel<kExactU8, false>(SearchArgs);
+template __global__ void search_kernel<kExactU8, true>(SearchArgs);
+template __global__ void search_kernel<kSeqF32, false>(SearchArgs);
+template __global__ void search_kernel<kSeqF32, true>(SearchArgs);
+template __global__ void search_kernel<kSeqU8, false>(SearchArgs);
+template __global__ void search_fast_kernel<kExactU8, 0>(SearchArgs);
+template __global__ void search_fast_kernel<kSeqF32, 0>(SearchArgs);
+template __global__ void search_fast_kernel<kSeqU8, 0>(SearchArgs);
+template __global__ void search_fast_kernel<kExactU8, 1>(SearchArgs);
+template __global__ void search_fast_kernel<kSeqF32, 1>(SearchArgs);
+template __global__ void search_fast_kernel<kExactU8, 2>(SearchArgs);
+template __global__ void search_fast_kernel<kSeqF32, 2>(SearchArgs);
 
 }  // namespace hm
 
@@ -1109,20 +1119,16 @@ namespace hm {
 using KernelFn = void (*)(SearchArgs);
 
 static KernelFn pick(int mode, bool filt, bool fast, int aux) {
-    if (aux == 1) return mode == kExactU8 ? search_kernel<kExactU8, false, true, 1>
-                                          : search_kernel<kSeqF32, false, true, 1>;
-    if (aux == 2) return mode == kExactU8 ? search_kernel<kExactU8, false, true, 2>
-                                          : search_kernel<kSeqF32, false, true, 2>;
+    if (aux == 1) return mode == kExactU8 ? search_fast_kernel<kExactU8, 1> : search_fast_kernel<kSeqF32, 1>;
+    if (aux == 2) return mode == kExactU8 ? search_fast_kernel<kExactU8, 2> : search_fast_kernel<kSeqF32, 2>;
     if (fast && !filt) {
-        if (mode == kExactU8) return search_kernel<kExactU8, false, true, 0>;
-        if (mode == kSeqF32) return search_kernel<kSeqF32, false, true, 0>;
-        return search_kernel<kSeqU8, false, true, 0>;
+        if (mode == kExactU8) return search_fast_kernel<kExactU8, 0>;
+        if (mode == kSeqF32) return search_fast_kernel<kSeqF32, 0>;
+        return search_fast_kernel<kSeqU8, 0>;
     }
-    if (mode == kExactU8)
-        return filt ? search_kernel<kExactU8, true, false, 0> : search_kernel<kExactU8, false, false, 0>;
-    if (mode == kSeqF32)
-        return filt ? search_kernel<kSeqF32, true, false, 0> : search_kernel<kSeqF32, false, false, 0>;
-    return search_kernel<kSeqU8, false, false, 0>;
+    if (mode == kExactU8) return filt ? search_kernel<kExactU8, true> : search_kernel<kExactU8, false>;
+    if (mode == kSeqF32) return filt ? search_kernel<kSeqF32, true> : search_kernel<kSeqF32, false>;
+    return search_kernel<kSeqU8, false>;
 }
 
 static int aux_of(const SearchArgs &a) { return a.trace ? 1 : (a.stop_w ? 2 : 0); }

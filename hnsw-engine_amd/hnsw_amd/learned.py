@@ -13,10 +13,11 @@ by the library's own forward/backward and Adam kernels, then a threshold chosen 
 queries for the recall target.
 
 Feature order (``features`` in search.hip): expansions so far, log2(1 + distances computed),
-ef, k, |R|/ef, unexpanded entries/ef, signed log2 of the best / k-th / worst / popped distance,
-(popped - best)/(|k-th - best| + 1), (k-th - best)/(|best| + 1), expansions since the k-th
-changed, accepted and new neighbours of the last expansion, log-ratio of the layer-0 entry
-distance to the best.
+ef, k, |R|/ef, unexpanded entries/ef, signed log2 of the best / k-th / next candidate's
+distance, (next - k-th)/(|k-th - best| + 1), (k-th - best)/(|best| + 1),
+(worst - k-th)/(|k-th - best| + 1), expansions since the k-th
+changed, accepted and new neighbours of the last expansion, entries of the current top-k not
+yet expanded. The search evaluates the model after every second expansion.
 """
 
 from __future__ import annotations
