@@ -2,28 +2,32 @@
 // masked brute force of benchmarks/run_filtered.py:76-106): the k smallest (distance, id)
 // pairs of each query over all (allowed) rows, distances in float64, ties to the lower id.
 //
-// Two distance producers feed one top-k consumer:
-//   * 8-bit integer data and queries (values 0..255, l2 or ip): int8 matrix cores. Values are
-//     shifted to -128..127; v_mfma_i32_32x32x32_i8 accumulates x'.q' exactly in int32 and
-//     |x-q|^2 = |x'|^2 + |q'|^2 - 2 x'.q' (ip: x.q = x'.q' + 128 (sum x' + sum q') + 128^2 dim)
-//     is an exact integer, so no error margin and no re-rank are needed: the float64 value of
-//     that integer is the reference's float64 distance bit for bit.
-//   * anything else: float64 on the vector units, each (query, row) pair summed left to right
-//     without fusion; cosine divides rows and queries by their float64 norms first
-//     (oracle.py:228-229).
-// Consumer: every work-group owns a block of queries and a chunk of rows and keeps, per query, a
-// small LDS buffer of candidates below a threshold (the k-th smallest (key, id) seen so far);
-// when a buffer nears capacity it is sorted and cut to k. Per-chunk top-k lists are merged per
-// query by a second kernel.
+// 8-bit integer data and queries (values 0..255, l2 or ip) use int8 matrix cores. Values are
+// shifted to -128..127; v_mfma_i32_32x32x32_i8 accumulates x'.q' exactly in int32 and
+// |x-q|^2 = |x'|^2 + |q'|^2 - 2 x'.q' (ip: x.q = x'.q' + 128 (sum x' + sum q') + 128^2 dim) is an
+// exact integer, so no error margin and no re-rank are needed: the float64 value of that integer
+// is the reference's float64 distance bit for bit. Two consumers:
+//   * the range search of exact_tile.hip (dim <= 256, k <= 128, >= 8192 rows): a sampled bound
+//     per query, then one scan that records only the rows the bound admits;
+//   * the chunked kernels below (every other case, and the queries the range search flags):
+//     every work-group owns a block of queries and a chunk of rows and keeps, per query, a small
+//     LDS buffer of candidates below a threshold (the k-th smallest (key, id) seen so far); when
+//     a buffer nears capacity it is sorted and cut to k. Per-chunk top-k lists are merged per
+//     query by a second kernel.
+// Anything else runs the chunked kernels with float64 on the vector units, each (query, row) pair
+// summed left to right without fusion; cosine divides rows and queries by their float64 norms
+// first (oracle.py:228-229).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "../../include/hnsw_mi355x.h"
 #include "common.hpp"
+#include "exact_tile.hpp"
 #include "index.hpp"
 #include "keys.hpp"
 #include "wave.hpp"
@@ -342,20 +346,38 @@ __global__ void normalise_kernel(const float *x, int64_t rows, int dim, double *
     for (int i = 0; i < dim; ++i) out[(size_t)r * dim + i] = (double)x[(size_t)r * dim + i] / nrm;
 }
 
-// 8-bit rows -> shifted int8 (zero padded to dimp) plus sum of squares and plain sum.
-__global__ void shift_i8_kernel(const float *x, int64_t rows, int dim, int dimp, int8_t *out,
-                                int32_t *sq, int32_t *sum) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// 8-bit rows -> shifted int8 (zero padded to dimp) plus sum of squares and plain sum. One wave
+// per row (four rows per 256-thread block), consecutive lanes on consecutive values.
+__global__ __launch_bounds__(256) void shift_i8_kernel(const float *x, int64_t rows, int dim,
+                                                       int dimp, int8_t *out, int32_t *sq,
+                                                       int32_t *sum) {
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= rows) return;
+    const int lane = lane_id();
     int32_t s2 = 0, s1 = 0;
-    for (int i = 0; i < dimp; ++i) {
-        int v = i < dim ? (int)x[(size_t)r * dim + i] - 128 : 0;
+    for (int i = lane; i < dimp; i += 64) {
+        const int v = i < dim ? (int)x[(size_t)r * dim + i] - 128 : 0;
         out[(size_t)r * dimp + i] = (int8_t)v;
         s2 += v * v;
         s1 += v;
     }
-    sq[r] = s2;
-    sum[r] = s1;
+    s2 = wave_sum(s2);
+    s1 = wave_sum(s1);
+    if (lane == 0) {
+        sq[r] = s2;
+        sum[r] = s1;
+    }
+}
+
+// Copies rows of (k int64 ids, k doubles) from the compact list back to their queries.
+__global__ void exact_scatter_kernel(const int64_t *sid, const double *sd, const int32_t *qidx,
+                                     int64_t m, int k, int64_t *ids, double *dists) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * k) return;
+    const int64_t r = e / k;
+    const int t = (int)(e % k);
+    ids[(int64_t)qidx[r] * k + t] = sid[e];
+    dists[(int64_t)qidx[r] * k + t] = sd[e];
 }
 
 }  // namespace hm
@@ -375,41 +397,57 @@ bool u8_values(const float *v, size_t count) {
 template <class T>
 struct Scoped {
     T *p = nullptr;
-    hipError_t alloc(size_t count) { return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T)); }
+    hipError_t alloc(size_t count) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
+    }
     ~Scoped() {
         if (p) (void)hipFree(p);
     }
 };
 
-}  // namespace
+// Rows of zero padding kept after the data so that any tile plan reads in bounds (a plan pads
+// each of at most kTileMaxChunks chunks by fewer than kTileRows rows).
+constexpr int kTileMaxChunks = 1024;
+constexpr int64_t kTilePadRows = (int64_t)kTileMaxChunks * kTileRows;
+constexpr int64_t kTileMinRows = 8192;     // smaller inputs go to the chunked kernels
+constexpr int kTileQueriesPerLaunch = 16384;
 
-extern "C" int hm_exact(const float *data, int64_t n, int32_t dim, int32_t metric,
-                        const float *queries, int64_t nq, int32_t k, const uint8_t *allowed,
-                        int device, int64_t *ids, double *dists) {
-    if (!data || !queries || !ids || !dists) return invalid("null argument");
-    if (n < 1 || dim < 1) return invalid("data must be non-empty with at least one dimension");
-    if (k < 1 || k > n) return invalid("k out of range");
-    if (metric < 0 || metric > 2) return invalid("unknown metric");
-    if (n >= (int64_t(1) << 31) - 1) return invalid("more than 2^31-2 rows");
+// Kernel times of the last range search on this thread (HIP events on its stream): bound pass
+// plus threshold, range pass, collect; and the number of queries re-run on the chunked kernels.
+struct ExactTiming {
+    double bound_ms = 0, range_ms = 0, collect_ms = 0;
+    int64_t redo = 0;
+};
+thread_local ExactTiming g_timing;
+
+struct Events {
+    hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+    ~Events() {
+        for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
+int pow2_at_least(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// The chunked kernels: per-chunk top-k lists in LDS, merged per query. Works for every k, dim
+// and metric; `a` holds the device inputs (int8 or float64 producer) and a.nq queries.
+int run_chunked(ExactArgs a, bool int8, const hipDeviceProp_t &prop, int64_t *d_ids,
+                double *d_dists, hipStream_t s) {
+    const int64_t n = a.n, nq = a.nq;
+    const int k = a.k;
     if (nq == 0) return HM_OK;
-    HM_TRY(hipSetDevice(device), "hipSetDevice");
-    hipDeviceProp_t prop;
-    HM_TRY(hipGetDeviceProperties(&prop, device), "device properties");
-    const bool int8 = metric != HM_COSINE && dim <= 1024 && u8_values(data, (size_t)n * dim) &&
-                      u8_values(queries, (size_t)nq * dim);
-    ExactArgs a;
-    memset(&a, 0, sizeof(a));
-    a.n = n;
-    a.dim = dim;
-    a.metric = metric;
-    a.k = k;
-    a.nq = (int)nq;
     // Queries per work-group: as many as the per-query buffers allow in LDS.
     int qb = 0, cb = 0, lds = 0;
     for (int cand : int8 ? std::vector<int>{64, 32} : std::vector<int>{64, 16}) {
         const int step_rows = int8 ? 32 : 4096 / cand;
-        int c = 1;
-        while (c < k + step_rows) c <<= 1;
+        const int c = pow2_at_least(k + step_rows);
         const int need = cand * c * 12 + cand * 4 + cand * 12 + 8 +
                          (int8 ? 0 : (cand + step_rows) * 16 * 8);
         if (need <= 160 * 1024) {
@@ -422,63 +460,22 @@ extern "C" int hm_exact(const float *data, int64_t n, int32_t dim, int32_t metri
     if (!qb) return invalid("k too large for the exact search");
     a.cb = cb;
     const int qblocks = (int)((nq + qb - 1) / qb);
-    int nchunks = (int)std::max<int64_t>(1, std::min<int64_t>((4 * prop.multiProcessorCount + qblocks - 1) / qblocks,
-                                                              (n + 2047) / 2048));
+    int nchunks = (int)std::max<int64_t>(
+        1, std::min<int64_t>((4 * prop.multiProcessorCount + qblocks - 1) / qblocks, (n + 2047) / 2048));
     nchunks = std::max(1, std::min(nchunks, 8192 / k));
     a.chunk = round_up((n + nchunks - 1) / nchunks, 64);
     nchunks = (int)((n + a.chunk - 1) / a.chunk);
     a.nchunks = nchunks;
     const int m = nchunks * k;
-    int p2 = 1;
-    while (p2 < m) p2 <<= 1;
+    const int p2 = pow2_at_least(m);
     if (p2 * 12 > 160 * 1024) return invalid("k too large for the merge");
-
-    Scoped<float> dx, dq;
-    Scoped<double> dxn, dqn;
-    Scoped<int8_t> x8, q8;
-    Scoped<int32_t> xsq, xsum, qsq, qsum;
-    Scoped<uint32_t> bits;
     Scoped<uint64_t> pk;
     Scoped<uint32_t> pi;
-    Scoped<int64_t> oid;
-    Scoped<double> od;
-    hipStream_t s = nullptr;
-    HM_TRY(dx.alloc((size_t)n * dim), "alloc");
-    HM_TRY(dq.alloc((size_t)nq * dim), "alloc");
-    HM_TRY(hipMemcpy(dx.p, data, (size_t)n * dim * 4, hipMemcpyHostToDevice), "copy data");
-    HM_TRY(hipMemcpy(dq.p, queries, (size_t)nq * dim * 4, hipMemcpyHostToDevice), "copy queries");
-    if (allowed) {
-        std::vector<uint32_t> b((size_t)(n + 31) / 32, 0u);
-        for (int64_t i = 0; i < n; ++i)
-            if (allowed[i]) b[i / 32] |= 1u << (i % 32);
-        HM_TRY(bits.alloc(b.size()), "alloc");
-        HM_TRY(hipMemcpy(bits.p, b.data(), b.size() * 4, hipMemcpyHostToDevice), "copy mask");
-        a.allowed = bits.p;
-    }
     HM_TRY(pk.alloc((size_t)nq * m), "alloc");
     HM_TRY(pi.alloc((size_t)nq * m), "alloc");
-    HM_TRY(oid.alloc((size_t)nq * k), "alloc");
-    HM_TRY(od.alloc((size_t)nq * k), "alloc");
     a.part_key = pk.p;
     a.part_id = pi.p;
     if (int8) {
-        a.dimp = (int)round_up(dim, 32);
-        HM_TRY(x8.alloc((size_t)n * a.dimp), "alloc");
-        HM_TRY(q8.alloc((size_t)nq * a.dimp), "alloc");
-        HM_TRY(xsq.alloc(n), "alloc");
-        HM_TRY(xsum.alloc(n), "alloc");
-        HM_TRY(qsq.alloc(nq), "alloc");
-        HM_TRY(qsum.alloc(nq), "alloc");
-        hipLaunchKernelGGL(shift_i8_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dx.p, n,
-                           dim, a.dimp, x8.p, xsq.p, xsum.p);
-        hipLaunchKernelGGL(shift_i8_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, dq.p,
-                           nq, dim, a.dimp, q8.p, qsq.p, qsum.p);
-        a.x8 = x8.p;
-        a.q8 = q8.p;
-        a.xsq = xsq.p;
-        a.xsum = xsum.p;
-        a.qsq = qsq.p;
-        a.qsum = qsum.p;
         const int nch = a.dimp / 32;
         void (*fn)(ExactArgs) = nch <= 4    ? exact_i8_kernel<4>
                                 : nch <= 8  ? exact_i8_kernel<8>
@@ -489,18 +486,6 @@ extern "C" int hm_exact(const float *data, int64_t n, int32_t dim, int32_t metri
                    "attribute");
         hipLaunchKernelGGL(fn, dim3(qblocks, nchunks), dim3(2 * qb), lds, s, a);
     } else {
-        a.x32 = dx.p;
-        a.q32 = dq.p;
-        if (metric == HM_COSINE) {
-            HM_TRY(dxn.alloc((size_t)n * dim), "alloc");
-            HM_TRY(dqn.alloc((size_t)nq * dim), "alloc");
-            hipLaunchKernelGGL(normalise_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dx.p,
-                               n, dim, dxn.p);
-            hipLaunchKernelGGL(normalise_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, dq.p,
-                               nq, dim, dqn.p);
-            a.xn = dxn.p;
-            a.qn = dqn.p;
-        }
         if (lds > 64 * 1024)
             HM_TRY(hipFuncSetAttribute((const void *)exact_f64_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds),
@@ -514,9 +499,322 @@ extern "C" int hm_exact(const float *data, int64_t n, int32_t dim, int32_t metri
                                    hipFuncAttributeMaxDynamicSharedMemorySize, mlds),
                "attribute");
     hipLaunchKernelGGL(exact_merge_kernel, dim3((unsigned)nq), dim3(64), mlds, s, pk.p, pi.p, nchunks,
-                       k, p2, int8 ? 1 : 0, metric, oid.p, od.p);
+                       k, p2, int8 ? 1 : 0, a.metric, d_ids, d_dists);
     HM_TRY(hipGetLastError(), "merge launch");
+    // The partial lists are freed on return: finish with them first.
+    HM_TRY(hipStreamSynchronize(s), "exact search");
+    return HM_OK;
+}
+
+// Geometry of one range-search launch over m rows and nqb queries.
+struct TilePlan {
+    int nch = 0, qb = 0, T = 0, groups = 0, nchunks = 0, sample_every = 1;
+    int E = 8;      // records per (query, chunk, half)
+    int cap = 0;    // candidates per query in the collect kernel
+    int64_t chunk = 0;
+};
+
+TilePlan plan_tile(int64_t m, int64_t nqb, int nch, int k) {
+    TilePlan p;
+    p.nch = nch;
+    const int qbmax = nch <= 4 ? 4 : 2;
+    p.qb = nqb <= 256 ? 1 : (nqb <= 512 || qbmax == 2 ? 2 : 4);
+    p.qb = std::min(p.qb, qbmax);
+    p.groups = (int)((nqb + 8 * p.qb * 32 - 1) / (8 * p.qb * 32));
+    // About four work-groups per compute unit of MI355X (256 CUs), chunks of >= 4096 rows.
+    p.nchunks = (int)std::max<int64_t>(1, std::min<int64_t>(1024 / p.groups, m / 4096));
+    p.nchunks = std::min(p.nchunks, kTileMaxChunks);
+    p.chunk = round_up((m + p.nchunks - 1) / p.nchunks, kTileRows);
+    p.nchunks = (int)((m + p.chunk - 1) / p.chunk);
+    const int nsb = (int)(p.chunk / kTileRows);
+    p.sample_every = std::max(1, std::min(16, nsb / 8));  // >= 8 sampled superblocks per chunk
+    p.T = k <= 32 ? 4 : 8;
+    // The threshold admits about k * sample_every rows per query (the k-th of a 1/sample_every
+    // sample; twice that for an unlucky sample), spread over 2 * nchunks record lists: room for
+    // twice the mean plus four standard deviations of a Poisson count.
+    const double mu = 2.0 * k * p.sample_every / (2.0 * p.nchunks);
+    p.E = (int)std::min(256.0, std::max(8.0, std::ceil(mu + 4.0 * std::sqrt(mu) + 4.0)));
+    p.cap = std::min(8192, std::max(1024, pow2_at_least(4 * k * p.sample_every)));
+    return p;
+}
+
+// The exact search with every buffer on the device. Integer data (int8): the range search over
+// matrix-core tiles (exact_tile.hip) when it applies, the chunked kernels otherwise and for the
+// queries it flags; anything else: the float64 chunked kernels.
+int exact_device(const float *d_data, int64_t n, int dim, int metric, const float *d_q,
+                 int64_t nq, int k, const uint32_t *d_bits, bool int8, int64_t *d_ids,
+                 double *d_dists, hipStream_t s, int device) {
+    g_timing = ExactTiming();
+    hipDeviceProp_t prop;
+    HM_TRY(hipGetDeviceProperties(&prop, device), "device properties");
+    ExactArgs a;
+    memset(&a, 0, sizeof(a));
+    a.n = n;
+    a.dim = dim;
+    a.metric = metric;
+    a.k = k;
+    a.nq = (int)nq;
+    a.allowed = d_bits;
+    if (!int8) {
+        Scoped<double> dxn, dqn;
+        a.x32 = d_data;
+        a.q32 = d_q;
+        if (metric == HM_COSINE) {
+            HM_TRY(dxn.alloc((size_t)n * dim), "alloc");
+            HM_TRY(dqn.alloc((size_t)nq * dim), "alloc");
+            hipLaunchKernelGGL(normalise_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                               d_data, n, dim, dxn.p);
+            hipLaunchKernelGGL(normalise_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s,
+                               d_q, nq, dim, dqn.p);
+            a.xn = dxn.p;
+            a.qn = dqn.p;
+        }
+        return run_chunked(a, false, prop, d_ids, d_dists, s);
+    }
+
+    const bool tile_ok = dim <= 256 && k <= 128 && n >= kTileMinRows;
+    const int nch = tile_ok ? pow2_at_least((dim + 31) / 32) : 0;
+    a.dimp = tile_ok ? 32 * nch : (int)round_up(dim, 32);
+    const int64_t rows_alloc = tile_ok ? n + kTilePadRows : n;
+    Scoped<int8_t> x8, q8;
+    Scoped<int32_t> xsq, xsum, qsq, qsum;
+    HM_TRY(x8.alloc((size_t)rows_alloc * a.dimp), "alloc");
+    HM_TRY(q8.alloc((size_t)nq * a.dimp), "alloc");
+    HM_TRY(xsq.alloc(rows_alloc), "alloc");
+    HM_TRY(xsum.alloc(rows_alloc), "alloc");
+    HM_TRY(qsq.alloc(nq), "alloc");
+    HM_TRY(qsum.alloc(nq), "alloc");
+    hipLaunchKernelGGL(shift_i8_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, d_data, n,
+                       dim, a.dimp, x8.p, xsq.p, xsum.p);
+    hipLaunchKernelGGL(shift_i8_kernel, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, d_q, nq,
+                       dim, a.dimp, q8.p, qsq.p, qsum.p);
+    HM_TRY(hipGetLastError(), "shift launch");
+    a.x8 = x8.p;
+    a.q8 = q8.p;
+    a.xsq = xsq.p;
+    a.xsum = xsum.p;
+    a.qsq = qsq.p;
+    a.qsum = qsum.p;
+    if (!tile_ok) return run_chunked(a, true, prop, d_ids, d_dists, s);
+    if (rows_alloc > n)
+        HM_TRY(hipMemsetAsync(x8.p + (size_t)n * a.dimp, 0, (size_t)(rows_alloc - n) * a.dimp, s),
+               "clear padding");
+
+    // Masked search: the allowed rows, compacted in id order.
+    int64_t m = n;
+    const int8_t *tx8 = x8.p;
+    const int32_t *txsq = xsq.p, *txsum = xsum.p;
+    Scoped<int32_t> idmap, cxsq, cxsum;
+    Scoped<int8_t> cx8;
+    Scoped<int64_t> dcount;
+    if (d_bits) {
+        HM_TRY(idmap.alloc(n), "alloc");
+        HM_TRY(dcount.alloc(1), "alloc");
+        HM_TRY(subset_ids_launch(d_bits, n, idmap.p, dcount.p, s), "subset launch");
+        HM_TRY(hipMemcpyAsync(&m, dcount.p, 8, hipMemcpyDeviceToHost, s), "copy count");
+        HM_TRY(hipStreamSynchronize(s), "subset count");
+        if (m < kTileMinRows || m < k) return run_chunked(a, true, prop, d_ids, d_dists, s);
+        HM_TRY(cx8.alloc((size_t)(m + kTilePadRows) * a.dimp), "alloc");
+        HM_TRY(cxsq.alloc(m + kTilePadRows), "alloc");
+        HM_TRY(cxsum.alloc(m + kTilePadRows), "alloc");
+        HM_TRY(hipMemsetAsync(cx8.p + (size_t)m * a.dimp, 0, (size_t)kTilePadRows * a.dimp, s),
+               "clear padding");
+        HM_TRY(gather_rows_launch(x8.p, xsq.p, xsum.p, idmap.p, m, a.dimp, cx8.p, cxsq.p, cxsum.p, s),
+               "gather launch");
+        tx8 = cx8.p;
+        txsq = cxsq.p;
+        txsum = cxsum.p;
+    }
+    Scoped<int32_t> cinit, flags;
+    HM_TRY(cinit.alloc(m + kTilePadRows), "alloc");
+    HM_TRY(flags.alloc(nq), "alloc");
+    HM_TRY(cinit_launch(txsq, txsum, m, m + kTilePadRows, metric, cinit.p, s), "cinit launch");
+
+    Scoped<int32_t> bnd, trig, rec_val, rec_row, rec_cnt;
+    size_t cap_bnd = 0, cap_rec = 0, cap_cnt = 0, cap_q = 0;
+    Events ev;
+    for (hipEvent_t &x : ev.e) HM_TRY(hipEventCreate(&x), "event");
+    double times[3] = {0, 0, 0};
+    for (int64_t b0 = 0; b0 < nq; b0 += kTileQueriesPerLaunch) {
+        const int64_t nqb = std::min<int64_t>(kTileQueriesPerLaunch, nq - b0);
+        const TilePlan p = plan_tile(m, nqb, nch, k);
+        const int per_query = p.nchunks * 2 * p.T;
+        if (per_query < 2 * k) return run_chunked(a, true, prop, d_ids, d_dists, s);
+        const size_t slots = (size_t)nqb * p.nchunks * 2;
+        if (slots * p.T > cap_bnd) {
+            HM_TRY(bnd.alloc(slots * p.T), "alloc");
+            cap_bnd = slots * p.T;
+        }
+        if (slots * p.E > cap_rec) {
+            HM_TRY(rec_val.alloc(slots * p.E * 16), "alloc");
+            HM_TRY(rec_row.alloc(slots * p.E), "alloc");
+            cap_rec = slots * p.E;
+        }
+        if (slots > cap_cnt) {
+            HM_TRY(rec_cnt.alloc(slots), "alloc");
+            cap_cnt = slots;
+        }
+        if ((size_t)nqb > cap_q) {
+            HM_TRY(trig.alloc(nqb), "alloc");
+            cap_q = nqb;
+        }
+        TileArgs t;
+        memset(&t, 0, sizeof(t));
+        t.x8 = tx8;
+        t.cinit = cinit.p;
+        t.q8 = q8.p + (size_t)b0 * a.dimp;
+        t.nq = (int)nqb;
+        t.chunk = p.chunk;
+        t.nchunks = p.nchunks;
+        t.sample_every = p.sample_every;
+        t.bnd = bnd.p;
+        t.trig = trig.p;
+        t.E = p.E;
+        t.rec_val = rec_val.p;
+        t.rec_row = rec_row.p;
+        t.rec_cnt = rec_cnt.p;
+        const dim3 grid(p.nchunks, p.groups);
+        HM_TRY(hipEventRecord(ev.e[0], s), "event");
+        HM_TRY(tile_launch(0, p.nch, p.qb, p.T, t, grid, s), "bound launch");
+        HM_TRY(threshold_launch(bnd.p, per_query, k, trig.p, (int)nqb, s), "threshold launch");
+        HM_TRY(hipEventRecord(ev.e[1], s), "event");
+        HM_TRY(tile_launch(1, p.nch, p.qb, p.T, t, grid, s), "range launch");
+        HM_TRY(hipEventRecord(ev.e[2], s), "event");
+        CollectArgs c;
+        memset(&c, 0, sizeof(c));
+        c.rec_val = rec_val.p;
+        c.rec_row = rec_row.p;
+        c.rec_cnt = rec_cnt.p;
+        c.trig = trig.p;
+        c.xsq = txsq;
+        c.idmap = d_bits ? idmap.p : nullptr;
+        c.qterm = (metric == HM_L2 ? qsq.p : qsum.p) + b0;
+        c.n = m;
+        c.nchunks = p.nchunks;
+        c.E = p.E;
+        c.k = k;
+        c.metric = metric;
+        c.dim = dim;
+        c.cap = p.cap;
+        c.ids = d_ids + b0 * k;
+        c.dists = d_dists + b0 * k;
+        c.flag = flags.p + b0;
+        HM_TRY(collect_launch(c, (int)nqb, s), "collect launch");
+        HM_TRY(hipEventRecord(ev.e[3], s), "event");
+        HM_TRY(hipEventSynchronize(ev.e[3]), "range search");
+        for (int i = 0; i < 3; ++i) {
+            float ms = 0.f;
+            HM_TRY(hipEventElapsedTime(&ms, ev.e[i], ev.e[i + 1]), "event time");
+            times[i] += ms;
+        }
+    }
+    g_timing.bound_ms = times[0];
+    g_timing.range_ms = times[1];
+    g_timing.collect_ms = times[2];
+    std::vector<int32_t> hflag((size_t)nq);
+    HM_TRY(hipMemcpyAsync(hflag.data(), flags.p, (size_t)nq * 4, hipMemcpyDeviceToHost, s), "copy flags");
+    HM_TRY(hipStreamSynchronize(s), "range search");
+    std::vector<int32_t> redo;
+    for (int64_t i = 0; i < nq; ++i)
+        if (hflag[(size_t)i]) redo.push_back((int32_t)i);
+    g_timing.redo = (int64_t)redo.size();
+    if (redo.empty()) return HM_OK;
+
+    // Flagged queries: the chunked kernels over the full rows (with the bitset when masked).
+    const int64_t nr = (int64_t)redo.size();
+    Scoped<int32_t> ridx, rqsq, rqsum;
+    Scoped<int8_t> rq8;
+    Scoped<int64_t> rids;
+    Scoped<double> rd;
+    HM_TRY(ridx.alloc(nr), "alloc");
+    HM_TRY(rq8.alloc((size_t)nr * a.dimp), "alloc");
+    HM_TRY(rqsq.alloc(nr), "alloc");
+    HM_TRY(rqsum.alloc(nr), "alloc");
+    HM_TRY(rids.alloc((size_t)nr * k), "alloc");
+    HM_TRY(rd.alloc((size_t)nr * k), "alloc");
+    HM_TRY(hipMemcpyAsync(ridx.p, redo.data(), (size_t)nr * 4, hipMemcpyHostToDevice, s), "copy list");
+    HM_TRY(gather_rows_launch(q8.p, qsq.p, qsum.p, ridx.p, nr, a.dimp, rq8.p, rqsq.p, rqsum.p, s),
+           "gather launch");
+    ExactArgs r = a;
+    r.q8 = rq8.p;
+    r.qsq = rqsq.p;
+    r.qsum = rqsum.p;
+    r.nq = (int)nr;
+    const int st = run_chunked(r, true, prop, rids.p, rd.p, s);
+    if (st != HM_OK) return st;
+    hipLaunchKernelGGL(exact_scatter_kernel, dim3((unsigned)((nr * k + 255) / 256)), dim3(256), 0, s,
+                       rids.p, rd.p, ridx.p, nr, k, d_ids, d_dists);
+    HM_TRY(hipGetLastError(), "scatter launch");
+    HM_TRY(hipStreamSynchronize(s), "exact search");
+    return HM_OK;
+}
+
+int check_exact(int64_t n, int32_t dim, int32_t metric, int32_t k) {
+    if (n < 1 || dim < 1) return invalid("data must be non-empty with at least one dimension");
+    if (k < 1 || k > n) return invalid("k out of range");
+    if (metric < 0 || metric > 2) return invalid("unknown metric");
+    if (n >= (int64_t(1) << 31) - 1 - kTilePadRows) return invalid("too many rows");
+    return HM_OK;
+}
+
+}  // namespace
+
+extern "C" int hm_exact(const float *data, int64_t n, int32_t dim, int32_t metric,
+                        const float *queries, int64_t nq, int32_t k, const uint8_t *allowed,
+                        int device, int64_t *ids, double *dists) {
+    if (!data || !queries || !ids || !dists) return invalid("null argument");
+    const int st = check_exact(n, dim, metric, k);
+    if (st != HM_OK) return st;
+    if (nq == 0) return HM_OK;
+    HM_TRY(hipSetDevice(device), "hipSetDevice");
+    const bool int8 = metric != HM_COSINE && dim <= 1024 && u8_values(data, (size_t)n * dim) &&
+                      u8_values(queries, (size_t)nq * dim);
+    Scoped<float> dx, dq;
+    Scoped<uint32_t> bits;
+    Scoped<int64_t> oid;
+    Scoped<double> od;
+    hipStream_t s = nullptr;
+    HM_TRY(dx.alloc((size_t)n * dim), "alloc");
+    HM_TRY(dq.alloc((size_t)nq * dim), "alloc");
+    HM_TRY(oid.alloc((size_t)nq * k), "alloc");
+    HM_TRY(od.alloc((size_t)nq * k), "alloc");
+    HM_TRY(hipMemcpy(dx.p, data, (size_t)n * dim * 4, hipMemcpyHostToDevice), "copy data");
+    HM_TRY(hipMemcpy(dq.p, queries, (size_t)nq * dim * 4, hipMemcpyHostToDevice), "copy queries");
+    if (allowed) {
+        std::vector<uint32_t> b((size_t)(n + 31) / 32, 0u);
+        for (int64_t i = 0; i < n; ++i)
+            if (allowed[i]) b[i / 32] |= 1u << (i % 32);
+        HM_TRY(bits.alloc(b.size()), "alloc");
+        HM_TRY(hipMemcpy(bits.p, b.data(), b.size() * 4, hipMemcpyHostToDevice), "copy mask");
+    }
+    const int rc = exact_device(dx.p, n, dim, metric, dq.p, nq, k, allowed ? bits.p : nullptr, int8,
+                                oid.p, od.p, s, device);
+    if (rc != HM_OK) return rc;
     HM_TRY(hipMemcpy(ids, oid.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost), "copy ids");
     HM_TRY(hipMemcpy(dists, od.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost), "copy dists");
+    return HM_OK;
+}
+
+extern "C" int hm_exact_dev(const float *d_data, int64_t n, int32_t dim, int32_t metric,
+                            const float *d_queries, int64_t nq, int32_t k,
+                            const uint32_t *d_allowed_bits, int exact_u8, int64_t *d_ids,
+                            double *d_dists, void *stream) {
+    if (!d_data || !d_queries || !d_ids || !d_dists) return invalid("null argument");
+    const int st = check_exact(n, dim, metric, k);
+    if (st != HM_OK) return st;
+    if (nq == 0) return HM_OK;
+    int device = 0;
+    HM_TRY(hipGetDevice(&device), "hipGetDevice");
+    const bool int8 = exact_u8 != 0 && metric != HM_COSINE && dim <= 1024;
+    return exact_device(d_data, n, dim, metric, d_queries, nq, k, d_allowed_bits, int8, d_ids,
+                        d_dists, (hipStream_t)stream, device);
+}
+
+extern "C" int hm_exact_timing(double *out) {
+    if (!out) return invalid("null argument");
+    out[0] = g_timing.bound_ms;
+    out[1] = g_timing.range_ms;
+    out[2] = g_timing.collect_ms;
+    out[3] = (double)g_timing.redo;
     return HM_OK;
 }

@@ -1,0 +1,56 @@
+// Shared definitions of the int8 matrix-core range search (exact_tile.hip) and its host driver
+// (exact.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hm {
+
+constexpr int kTileRows = 64;      // rows per LDS superblock (two 32-row MFMA blocks)
+constexpr int kTileThreads = 512;  // 8 waves: two per SIMD
+constexpr int32_t kTilePadInit = -(1 << 30);
+
+struct TileArgs {
+    const int8_t *x8;      // (nchunks * chunk, 32 * NCH) shifted rows, zero padding
+    const int32_t *cinit;  // (nchunks * chunk) accumulator start per row
+    const int8_t *q8;      // (nq, 32 * NCH) shifted queries
+    int nq;
+    int64_t chunk;  // rows per chunk, a multiple of kTileRows
+    int nchunks;
+    int sample_every;  // bound pass: superblocks 0, s, 2s, ... of each chunk
+    int32_t *bnd;      // bound pass out: (nq, nchunks, 2, T)
+    const int32_t *trig;  // range pass: (nq) V
+    int E;                // records per (query, chunk, half)
+    int32_t *rec_val;     // (nq, nchunks, 2, E, 16) scores
+    int32_t *rec_row;     // (nq, nchunks, 2, E) first row of the 32-row block
+    int32_t *rec_cnt;     // (nq, nchunks, 2) records written (> E: overflow)
+};
+
+struct CollectArgs {
+    const int32_t *rec_val, *rec_row, *rec_cnt, *trig;
+    const int32_t *xsq;    // l2: |x'|^2 per row
+    const int32_t *idmap;  // row -> id, or null (identity)
+    const int32_t *qterm;  // per query: |q'|^2 (l2) or sum(q') (ip)
+    int64_t n;             // valid rows
+    int nchunks, E, k, metric, dim, cap;
+    int64_t *ids;
+    double *dists;
+    int32_t *flag;  // 1: re-run this query on the general kernel
+};
+
+// Host launchers (exact_tile.hip); each returns the launch status. The tile grid is
+// (nchunks, query groups of 8 * qb * 32).
+hipError_t tile_launch(int mode, int nch, int qb, int T, const TileArgs &a, dim3 grid,
+                       hipStream_t s);
+hipError_t threshold_launch(const int32_t *bnd, int per_query, int k, int32_t *trig, int nq,
+                            hipStream_t s);
+hipError_t collect_launch(const CollectArgs &a, int nq, hipStream_t s);
+hipError_t cinit_launch(const int32_t *xsq, const int32_t *xsum, int64_t n, int64_t rows,
+                        int metric, int32_t *cinit, hipStream_t s);
+hipError_t subset_ids_launch(const uint32_t *bits, int64_t n, int32_t *idmap, int64_t *count,
+                             hipStream_t s);
+hipError_t gather_rows_launch(const int8_t *x8, const int32_t *xsq, const int32_t *xsum,
+                              const int32_t *idmap, int64_t m, int dimp, int8_t *ox8,
+                              int32_t *oxsq, int32_t *oxsum, hipStream_t s);
+
+}  // namespace hm

@@ -25,6 +25,8 @@ SYMBOLS = (
     "hm_search_codes",
     "hm_rerank",
     "hm_exact",
+    "hm_exact_dev",
+    "hm_exact_timing",
     "hm_build",
     "hm_model_size",
     "hm_set_stop_model",
@@ -109,6 +111,9 @@ def lib() -> ctypes.CDLL:
             "hm_search_codes": ([vp, vp, ctypes.POINTER(SearchOpts), vp, vp, vp, vp], ctypes.c_int),
             "hm_rerank": ([vp, vp, i64, vp, i32, i32, vp, vp], ctypes.c_int),
             "hm_exact": ([vp, i64, i32, i32, vp, i64, i32, vp, ctypes.c_int, vp, vp], ctypes.c_int),
+            "hm_exact_dev": ([vp, i64, i32, i32, vp, i64, i32, vp, ctypes.c_int, vp, vp, vp],
+                             ctypes.c_int),
+            "hm_exact_timing": ([vp], ctypes.c_int),
             "hm_build": (
                 [i64, i64, i32, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp],
                 ctypes.c_int,

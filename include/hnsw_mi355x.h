@@ -4,8 +4,8 @@
  * Every entry point takes plain pointers and sizes and returns 0 on success or a negative
  * status; hm_last_error() then holds the message (the Python layer re-raises it as the
  * reference's exception type). Host-pointer entry points copy to and from HBM themselves;
- * the *_dev variants take device pointers and a HIP stream and allocate nothing per call
- * beyond the scratch the handle keeps between calls.
+ * the *_dev variants take device pointers and a HIP stream; the search ones allocate nothing
+ * per call beyond the scratch the handle keeps between calls.
  *
  * Reference interfaces replaced (paths relative to the hnsw-engine repository):
  *   hm_open            NumbaEngine.attach_native      src/hnsw_engine/numba_engine.py:827-868
@@ -17,7 +17,7 @@
  *   hm_rerank          NumbaEngine.rerank             src/hnsw_engine/numba_engine.py:703-713
  *                      (-> _rerank_batch :506-530)
  *   hm_exact           exact_search                   src/hnsw_engine/oracle.py:35-126
- *                      (masked: exact_subset_search   benchmarks/run_filtered.py:76-106)
+ *   hm_exact_dev       (masked: exact_subset_search   benchmarks/run_filtered.py:76-106)
  *   hm_build           NumbaEngine.insert             src/hnsw_engine/numba_engine.py:609-643
  *                      (-> _insert_batch :287-419; host C++, not a GPU kernel)
  */
@@ -134,6 +134,22 @@ int hm_rerank(hm_index *index, const float *queries, int64_t nq, const int64_t *
 int hm_exact(const float *data, int64_t n, int32_t dim, int32_t metric, const float *queries,
              int64_t nq, int32_t k, const uint8_t *allowed, int device, int64_t *ids,
              double *dists);
+
+/* hm_exact with every buffer already in HBM, on the current device: d_data float32 (n, dim),
+ * d_queries float32 (nq, dim), d_allowed_bits ceil(n/32) uint32 words (bit i of word i/32 set:
+ * row i allowed) or null, outputs int64 / float64 (nq, k). exact_u8 = 1 selects the int8
+ * matrix-core path: the caller guarantees that every data and query value is an integer in
+ * 0..255 (metric l2 or ip); 0 selects the float64 path. Scratch is allocated per call and the
+ * call synchronises `stream` (a hipStream_t, null: the default stream) before it returns. */
+int hm_exact_dev(const float *d_data, int64_t n, int32_t dim, int32_t metric,
+                 const float *d_queries, int64_t nq, int32_t k, const uint32_t *d_allowed_bits,
+                 int exact_u8, int64_t *d_ids, double *d_dists, void *stream);
+
+/* Instrumentation (no counterpart in the reference): kernel times in ms of the last exact
+ * search on the calling thread that took the int8 range-search path, out[0] bound pass plus
+ * threshold, out[1] range pass, out[2] collect, and out[3] the number of queries it re-ran on
+ * the chunked kernels (all 0 when the last search took another path). */
+int hm_exact_timing(double *out);
 
 /* Sequential insertion (host C++, the reference's _insert_batch): inserts vectors[start..n)
  * with their levels into the native arrays, which hold capacity for all n nodes and
