@@ -520,14 +520,17 @@ TilePlan plan_tile(int64_t m, int64_t nqb, int nch, int k) {
     const int qbmax = nch <= 4 ? 4 : 2;
     p.qb = nqb <= 256 ? 1 : (nqb <= 512 || qbmax == 2 ? 2 : 4);
     p.qb = std::min(p.qb, qbmax);
-    p.groups = (int)((nqb + 8 * p.qb * 32 - 1) / (8 * p.qb * 32));
+    p.groups = (int)((nqb + kTileWaves * p.qb * 32 - 1) / (kTileWaves * p.qb * 32));
     // About four work-groups per compute unit of MI355X (256 CUs), chunks of >= 4096 rows.
     p.nchunks = (int)std::max<int64_t>(1, std::min<int64_t>(1024 / p.groups, m / 4096));
     p.nchunks = std::min(p.nchunks, kTileMaxChunks);
     p.chunk = round_up((m + p.nchunks - 1) / p.nchunks, kTileRows);
     p.nchunks = (int)((m + p.chunk - 1) / p.chunk);
     const int nsb = (int)(p.chunk / kTileRows);
-    p.sample_every = std::max(1, std::min(16, nsb / 8));  // >= 8 sampled superblocks per chunk
+    // A quarter of each chunk (at least two superblocks): measured on 10,000 x 1M x 128, a
+    // sixteenth makes the bound pass 2x cheaper but the looser threshold costs more in the range
+    // pass (more 16-row groups reach it) and in the collect.
+    p.sample_every = std::max(1, std::min(4, nsb / 2));
     p.T = k <= 32 ? 4 : 8;
     // The threshold admits about k * sample_every rows per query (the k-th of a 1/sample_every
     // sample; twice that for an unlucky sample), spread over 2 * nchunks record lists: room for
@@ -674,8 +677,13 @@ int exact_device(const float *d_data, int64_t n, int dim, int metric, const floa
         t.rec_row = rec_row.p;
         t.rec_cnt = rec_cnt.p;
         const dim3 grid(p.nchunks, p.groups);
+        // The bound pass with eight list entries per query block keeps fewer query blocks per
+        // wave at 128 dimensions (register budget); its output does not depend on the grouping.
+        const int qb_bound = (p.T == 8 && p.nch >= 4) ? std::min(p.qb, 2) : p.qb;
+        const dim3 grid_bound(p.nchunks, (unsigned)((nqb + kTileWaves * qb_bound * 32 - 1) /
+                                                    (kTileWaves * qb_bound * 32)));
         HM_TRY(hipEventRecord(ev.e[0], s), "event");
-        HM_TRY(tile_launch(0, p.nch, p.qb, p.T, t, grid, s), "bound launch");
+        HM_TRY(tile_launch(0, p.nch, qb_bound, p.T, t, grid_bound, s), "bound launch");
         HM_TRY(threshold_launch(bnd.p, per_query, k, trig.p, (int)nqb, s), "threshold launch");
         HM_TRY(hipEventRecord(ev.e[1], s), "event");
         HM_TRY(tile_launch(1, p.nch, p.qb, p.T, t, grid, s), "range launch");

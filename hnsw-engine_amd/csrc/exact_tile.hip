@@ -7,7 +7,7 @@
 // needs is a threshold per query that is known to admit the true k nearest before the scan
 // starts, so the scan keeps no per-query state beyond one compare. Three kernels:
 //
-//   bound     a sample of the rows (every sample_every-th 64-row superblock of each chunk):
+//   bound     a sample of the rows (every sample_every-th 128-row superblock of each chunk):
 //             per lane, the T largest "block scores" (the best score among the 16 rows a lane
 //             holds for one 32-row MFMA block);
 //   threshold per query, V = the k-th largest of those block scores. They belong to disjoint
@@ -27,9 +27,10 @@
 //
 // Tiling: a work-group is 8 waves (two per SIMD, so one wave's matrix-core work overlaps the
 // other's compares) over QB x 32 queries each (B operands in registers for the whole launch) and
-// one chunk of rows, staged 64 rows at a time through double-buffered LDS (XOR-swizzled 16-byte
-// slots, so the 32 lanes reading 32 rows hit distinct banks). Per 32-row block and wave: 4 QB
-// MFMAs with NCH K-steps, then a max over each lane's 16 scores and one compare.
+// one chunk of rows, staged 128 rows at a time through double-buffered LDS (XOR-swizzled 16-byte
+// slots, so the 32 lanes reading 32 rows hit distinct banks); one barrier per 128 rows. Per
+// 32-row block and wave: QB x NCH MFMAs (the next block's operands load meanwhile), then a max
+// over each lane's 16 scores and one compare per query block.
 #include <hip/hip_runtime.h>
 #include <limits.h>
 #include <stdint.h>
@@ -56,26 +57,33 @@ HM_DEV int32_t max16(const i32x16 &v) {
 
 // MODE 0: bound pass (sampled superblocks, top-T block scores per lane); MODE 1: range pass.
 template <int MODE, int NCH, int QB, int T>
-__global__ __launch_bounds__(kTileThreads) void exact_tile_kernel(TileArgs a) {
+__global__ __launch_bounds__(kTileWaves * 64) void exact_tile_kernel(TileArgs a) {
+    constexpr int WPB = kTileWaves;
+    constexpr int kTileThreads = WPB * 64;
     constexpr int ROWB = NCH * 32;
     constexpr int SLOTS = 2 * NCH;
-    constexpr int ELEMS = kTileRows * SLOTS;
+    // Superblock: 32 KB of rows (128 or 256 rows), so that one superblock of prefetch covers
+    // the memory latency; kTileRows (256) is a multiple of every size.
+    constexpr int SBR = NCH >= 8 ? 128 : 256;
+    constexpr int SUBS = SBR / 32;
+    static_assert(kTileRows % SBR == 0, "chunks are multiples of every superblock size");
+    constexpr int ELEMS = SBR * SLOTS;
     constexpr int LPT = (ELEMS + kTileThreads - 1) / kTileThreads;
-    __shared__ __attribute__((aligned(16))) uint8_t rows_lds[2][kTileRows * ROWB];
-    __shared__ __attribute__((aligned(16))) int32_t ci_lds[2][kTileRows];
+    __shared__ __attribute__((aligned(16))) uint8_t rows_lds[2][SBR * ROWB];
+    __shared__ __attribute__((aligned(16))) int32_t ci_lds[2][SBR];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int half = lane >> 5, col = lane & 31;
     const int chunk = blockIdx.x;
     const int64_t r0 = (int64_t)chunk * a.chunk;
-    const int nsb_all = (int)(a.chunk / kTileRows);
+    const int nsb_all = (int)(a.chunk / SBR);
     const int nsb = MODE == 0 ? (nsb_all + a.sample_every - 1) / a.sample_every : nsb_all;
 
     int qi[QB];
     i32x4 bq[QB][NCH];
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb) {
-        qi[qb] = blockIdx.y * (8 * QB * 32) + (wave * QB + qb) * 32 + col;
+        qi[qb] = blockIdx.y * (WPB * QB * 32) + (wave * QB + qb) * 32 + col;
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             bq[qb][c] = i32x4{0, 0, 0, 0};
@@ -95,69 +103,85 @@ __global__ __launch_bounds__(kTileThreads) void exact_tile_kernel(TileArgs a) {
         for (int t = 0; t < T; ++t) top[qb][t] = INT_MIN;
     }
 
-    uint4 pre[LPT];
+    // Prefetch registers of the next superblock (plain variables: a private array captured by a
+    // lambda stays in scratch memory, and the store to it would wait for the load).
+    static_assert(LPT <= 4, "at most four 16-byte loads per thread and superblock");
+    uint4 pre0 = uint4{0, 0, 0, 0}, pre1 = pre0, pre2 = pre0, pre3 = pre0;
     int4 pci = int4{0, 0, 0, 0};
-    auto load = [&](int s) {
-        const int sb = MODE == 0 ? s * a.sample_every : s;
-        const int64_t base = r0 + (int64_t)sb * kTileRows;
-#pragma unroll
-        for (int t = 0; t < LPT; ++t) {
-            const int e = tid + t * kTileThreads;
-            if (e < ELEMS)
-                pre[t] = *(const uint4 *)(a.x8 + (size_t)(base + e / SLOTS) * ROWB + (e % SLOTS) * 16);
-        }
-        if (tid < kTileRows / 4) pci = *(const int4 *)(a.cinit + base + tid * 4);
+#define HM_TILE_LOAD(S)                                                                       \
+    do {                                                                                      \
+        const int sb_ = MODE == 0 ? (S) * a.sample_every : (S);                               \
+        const int64_t base_ = r0 + (int64_t)sb_ * SBR;                                        \
+        const int8_t *src_ = a.x8 + (size_t)base_ * ROWB;                                     \
+        if (tid < ELEMS)                                                                      \
+            pre0 = *(const uint4 *)(src_ + (size_t)(tid / SLOTS) * ROWB + (tid % SLOTS) * 16); \
+        if (LPT > 1 && tid + kTileThreads < ELEMS)                                            \
+            pre1 = *(const uint4 *)(src_ + (size_t)((tid + kTileThreads) / SLOTS) * ROWB +     \
+                                    ((tid + kTileThreads) % SLOTS) * 16);                     \
+        if (LPT > 2 && tid + 2 * kTileThreads < ELEMS)                                        \
+            pre2 = *(const uint4 *)(src_ + (size_t)((tid + 2 * kTileThreads) / SLOTS) * ROWB + \
+                                    ((tid + 2 * kTileThreads) % SLOTS) * 16);                 \
+        if (LPT > 3 && tid + 3 * kTileThreads < ELEMS)                                        \
+            pre3 = *(const uint4 *)(src_ + (size_t)((tid + 3 * kTileThreads) / SLOTS) * ROWB + \
+                                    ((tid + 3 * kTileThreads) % SLOTS) * 16);                 \
+        if (tid < SBR / 4) pci = *(const int4 *)(a.cinit + base_ + tid * 4);                  \
+    } while (0)
+    auto stash_one = [&](int buf, int e, uint4 v) {
+        const int r = e / SLOTS, j = e % SLOTS;
+        const int p = (j ^ tile_swz(r)) & (SLOTS - 1);
+        *(uint4 *)(rows_lds[buf] + r * ROWB + p * 16) = v;
     };
     auto stash = [&](int buf) {
+        if (tid < ELEMS) stash_one(buf, tid, pre0);
+        if (LPT > 1 && tid + kTileThreads < ELEMS) stash_one(buf, tid + kTileThreads, pre1);
+        if (LPT > 2 && tid + 2 * kTileThreads < ELEMS) stash_one(buf, tid + 2 * kTileThreads, pre2);
+        if (LPT > 3 && tid + 3 * kTileThreads < ELEMS) stash_one(buf, tid + 3 * kTileThreads, pre3);
+        if (tid < SBR / 4) *(int4 *)(ci_lds[buf] + tid * 4) = pci;
+    };
+    // Operands of one 32-row block: this lane's row (A) and the accumulator start of the 16
+    // rows whose scores it will hold.
+    auto operands = [&](int buf, int h2, i32x4 *af, i32x16 &ci) {
+        const int r = h2 * 32 + col;
 #pragma unroll
-        for (int t = 0; t < LPT; ++t) {
-            const int e = tid + t * kTileThreads;
-            if (e < ELEMS) {
-                const int r = e / SLOTS, j = e % SLOTS;
-                const int p = (j ^ tile_swz(r)) & (SLOTS - 1);
-                *(uint4 *)(rows_lds[buf] + r * ROWB + p * 16) = pre[t];
-            }
+        for (int c = 0; c < NCH; ++c) {
+            const int p = ((2 * c + half) ^ tile_swz(r)) & (SLOTS - 1);
+            af[c] = *(const i32x4 *)(rows_lds[buf] + r * ROWB + p * 16);
         }
-        if (tid < kTileRows / 4) *(int4 *)(ci_lds[buf] + tid * 4) = pci;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int4 v = *(const int4 *)(ci_lds[buf] + h2 * 32 + 8 * g + 4 * half);
+            ci[4 * g + 0] = v.x;
+            ci[4 * g + 1] = v.y;
+            ci[4 * g + 2] = v.z;
+            ci[4 * g + 3] = v.w;
+        }
     };
 
     // The query operands and thresholds must have landed before the loop: otherwise the wait the
     // compiler places before the first matrix op of every iteration also waits for the next
     // superblock's prefetch (the memory counter is in order), exposing its latency.
     __builtin_amdgcn_s_waitcnt(0);
-    if (nsb > 0) load(0);
+    if (nsb > 0) HM_TILE_LOAD(0);
     for (int s = 0; s < nsb; ++s) {
         const int buf = s & 1;
         stash(buf);
         __syncthreads();
-        if (s + 1 < nsb) load(s + 1);
+        if (s + 1 < nsb) HM_TILE_LOAD(s + 1);
         const int sb = MODE == 0 ? s * a.sample_every : s;
+        // The next block's operands load while this block's matrix ops run.
+        i32x4 af[NCH];
+        i32x16 ci;
+        operands(buf, 0, af, ci);
 #pragma unroll
-        for (int h2 = 0; h2 < 2; ++h2) {
-            const int r = h2 * 32 + col;
-            i32x4 af[NCH];
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                const int p = ((2 * c + half) ^ tile_swz(r)) & (SLOTS - 1);
-                af[c] = *(const i32x4 *)(rows_lds[buf] + r * ROWB + p * 16);
-            }
-            i32x16 ci;
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const int4 v = *(const int4 *)(ci_lds[buf] + h2 * 32 + 8 * g + 4 * half);
-                ci[4 * g + 0] = v.x;
-                ci[4 * g + 1] = v.y;
-                ci[4 * g + 2] = v.z;
-                ci[4 * g + 3] = v.w;
-            }
+        for (int h2 = 0; h2 < SUBS; ++h2) {
             i32x16 acc[QB];
-#pragma unroll
-            for (int qb = 0; qb < QB; ++qb) acc[qb] = ci;
 #pragma unroll
             for (int c = 0; c < NCH; ++c)
 #pragma unroll
                 for (int qb = 0; qb < QB; ++qb)
-                    acc[qb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[c], bq[qb][c], acc[qb], 0, 0, 0);
+                    acc[qb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[c], bq[qb][c],
+                                                                    c == 0 ? ci : acc[qb], 0, 0, 0);
+            if (h2 + 1 < SUBS) operands(buf, h2 + 1, af, ci);
             if constexpr (MODE == 0) {
 #pragma unroll
                 for (int qb = 0; qb < QB; ++qb) {
@@ -178,7 +202,7 @@ __global__ __launch_bounds__(kTileThreads) void exact_tile_kernel(TileArgs a) {
                     any |= m[qb] >= V[qb];
                 }
                 if (ballot(any)) {
-                    const int32_t row0 = (int32_t)(r0 + (int64_t)sb * kTileRows + h2 * 32);
+                    const int32_t row0 = (int32_t)(r0 + (int64_t)sb * SBR + h2 * 32);
 #pragma unroll
                     for (int qb = 0; qb < QB; ++qb) {
                         if (m[qb] >= V[qb]) {
@@ -210,6 +234,7 @@ __global__ __launch_bounds__(kTileThreads) void exact_tile_kernel(TileArgs a) {
             a.rec_cnt[slot] = cnt[qb];
         }
     }
+#undef HM_TILE_LOAD
 }
 
 // V for each query: the k-th largest of its per_query sampled block scores (one wave per
@@ -406,7 +431,7 @@ __global__ void exact_gather_rows_kernel(const int8_t *x8, const int32_t *xsq,
 
 template <int MODE, int NCH, int QB, int T>
 static hipError_t launch_tile(const TileArgs &a, dim3 grid, hipStream_t s) {
-    hipLaunchKernelGGL((exact_tile_kernel<MODE, NCH, QB, T>), grid, dim3(kTileThreads), 0, s, a);
+    hipLaunchKernelGGL((exact_tile_kernel<MODE, NCH, QB, T>), grid, dim3(kTileWaves * 64), 0, s, a);
     return hipGetLastError();
 }
 

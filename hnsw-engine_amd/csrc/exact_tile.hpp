@@ -6,8 +6,8 @@
 
 namespace hm {
 
-constexpr int kTileRows = 64;      // rows per LDS superblock (two 32-row MFMA blocks)
-constexpr int kTileThreads = 512;  // 8 waves: two per SIMD
+constexpr int kTileRows = 256;     // chunks are multiples of this many rows (the largest superblock)
+constexpr int kTileWaves = 8;      // waves per work-group: two per SIMD
 constexpr int32_t kTilePadInit = -(1 << 30);
 
 struct TileArgs {
