@@ -59,6 +59,9 @@ struct Batch {
 };
 
 constexpr int64_t kDefaultSpill = 16384;
+// Ids per wave slot remembered for clearing its visited bitset (a layer search that sets more
+// clears the whole bitset).
+constexpr int kVisitedList = 4096;
 
 // Capacities of the on-chip state for a batch whose largest max(ef, k) is max_ef.
 void plan(SearchArgs &a, int mode, const hm_index *h, int max_ef, bool fast) {
@@ -142,6 +145,9 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, 
     a.bitset_words = (h->n + 31) / 32;
     HM_TRY(h->bits.ensure((size_t)slots * a.bitset_words, true, s), "bitset allocation");
     a.bitsets = h->bits.p;
+    a.vlist_cap = kVisitedList;
+    HM_TRY(h->vlist.ensure((size_t)slots * kVisitedList), "visited list allocation");
+    a.vlist = h->vlist.p;
     a.spill_cap = spill_cap;
     HM_TRY(h->spill_key.ensure((size_t)slots * spill_cap), "spill allocation");
     HM_TRY(h->spill_id.ensure((size_t)slots * spill_cap), "spill allocation");

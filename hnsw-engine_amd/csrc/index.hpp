@@ -18,7 +18,7 @@ struct hm_index {
     hm::DevBuf<float> vec;        // (n, dim)
     hm::DevBuf<uint8_t> u8, codes;  // (n, dimp)
     // search scratch
-    hm::DevBuf<uint32_t> bits;
+    hm::DevBuf<uint32_t> bits, vlist;
     hm::DevBuf<uint64_t> spill_key;
     hm::DevBuf<uint32_t> spill_id;
     hm::DevBuf<int32_t> queue;
@@ -47,6 +47,7 @@ struct hm_index {
         u8.release();
         codes.release();
         bits.release();
+        vlist.release();
         spill_key.release();
         spill_id.release();
         q32.release();

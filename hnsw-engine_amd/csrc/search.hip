@@ -267,18 +267,42 @@ HM_DEV double decode(uint64_t k, int metric) {
 struct Visited {
     int32_t *stats;
     uint32_t *bits;
+    uint32_t *list;  // ids set in the bitset during this layer search (for clearing)
+    int list_cap, list_n;  // list_n > list_cap: too many, clear the whole bitset
     int hlog;    // active hash size is 1 << hlog
     int count;   // ids inserted in this layer
     bool use_bits, dirty;
 };
 
+// Zeroes the bitset words this layer search set: through the id list when it held them all,
+// otherwise the whole bitset.
+HM_DEV void visited_clear(const SearchArgs &a, Visited &V) {
+    const int lane = lane_id();
+    if (V.list_n <= V.list_cap) {
+        for (int i = lane; i < V.list_n; i += 64) V.bits[V.list[i] >> 5] = 0u;
+    } else {
+        for (int64_t w = lane; w < a.bitset_words; w += 64) V.bits[w] = 0u;
+    }
+    __threadfence();
+    V.dirty = false;
+    V.list_n = 0;
+}
+
+// Appends the ids of the lanes with `isnew` to the clearing list (bitset mode only).
+HM_DEV void visited_note(Visited &V, bool isnew, uint32_t id) {
+    if (!V.use_bits) return;
+    const uint64_t m = ballot(isnew);
+    if (V.list_n + popc(m) <= V.list_cap) {
+        if (isnew) V.list[V.list_n + rank_below(m)] = id;
+        V.list_n += popc(m);
+    } else {
+        V.list_n = V.list_cap + 1;
+    }
+}
+
 HM_DEV void visited_reset(const SearchArgs &a, const Lds &L, Visited &V, int hlog) {
     const int lane = lane_id();
-    if (V.dirty) {
-        for (int64_t w = lane; w < a.bitset_words; w += 64) V.bits[w] = 0u;
-        __threadfence();
-        V.dirty = false;
-    }
+    if (V.dirty) visited_clear(a, V);
     V.use_bits = false;
     V.hlog = hlog;
     V.count = 0;
@@ -290,13 +314,15 @@ HM_DEV void visited_reset(const SearchArgs &a, const Lds &L, Visited &V, int hlo
 HM_DEV void visited_make_room(const Lds &L, Visited &V, int incoming) {
     if (V.use_bits || V.count + incoming <= (1 << V.hlog) / 2) return;
     const int lane = lane_id();
-    for (int i = lane; i < (1 << V.hlog); i += 64) {
-        uint32_t v = L.hash[i];
-        if (v) atomicOr(&V.bits[(v - 1) >> 5], 1u << ((v - 1) & 31));
-    }
-    __threadfence();
     V.use_bits = true;
     V.dirty = true;
+    V.list_n = 0;
+    for (int i0 = 0; i0 < (1 << V.hlog); i0 += 64) {
+        const uint32_t v = L.hash[i0 + lane];
+        if (v) atomicOr(&V.bits[(v - 1) >> 5], 1u << ((v - 1) & 31));
+        visited_note(V, v != 0u, v - 1u);
+    }
+    __threadfence();
     if (lane == 0 && V.stats) atomicAdd(&V.stats[0], 1);
 }
 
@@ -350,6 +376,7 @@ HM_DEV void greedy_layer(const SearchArgs &a, const Lds &L, Visited &V, Query &Q
             const int32_t nb = j < a.w1 ? a.links_up[row * a.w1 + j] : -1;
             visited_make_room(L, V, 64);
             const bool isnew = nb >= 0 && visit(L, V, (uint32_t)nb);
+            visited_note(V, isnew, (uint32_t)nb);
             const uint64_t m = ballot(isnew);
             const int nnew = popc(m);
             if (nnew == 0) continue;
@@ -548,6 +575,7 @@ HM_DEV bool layer0(const SearchArgs &a, const Lds &L, Visited &V, Query &Q, int 
             const int32_t nb = j < a.w0 ? a.links0[(size_t)cid * a.w0 + j] : -1;
             visited_make_room(L, V, 64);
             const bool isnew = nb >= 0 && visit(L, V, (uint32_t)nb);
+            visited_note(V, isnew, (uint32_t)nb);
             const uint64_t nm = ballot(isnew);
             const int nnew = popc(nm);
             if (nnew == 0) continue;
@@ -824,6 +852,7 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
             const int32_t nb = j < a.w0 ? a.links0[(size_t)cid * a.w0 + j] : -1;
             visited_make_room(L, V, min(64, a.w0 - base));
             const bool isnew = nb >= 0 && visit(L, V, (uint32_t)nb);
+            visited_note(V, isnew, (uint32_t)nb);
             const uint64_t nm = ballot(isnew);
             const int nnew = popc(nm);
             if (nnew == 0) continue;
@@ -1062,6 +1091,9 @@ HM_DEV void search_loop(const SearchArgs &a) {
     const Lds L = FAST ? carve_fast(lds_raw, a, MODE) : carve(lds_raw, a, MODE);
     Visited V;
     V.bits = a.bitsets + (size_t)slot * a.bitset_words;
+    V.list = a.vlist + (size_t)slot * a.vlist_cap;
+    V.list_cap = a.vlist_cap;
+    V.list_n = 0;
     V.stats = a.stats;
     V.dirty = false;
     V.use_bits = false;
@@ -1077,9 +1109,7 @@ HM_DEV void search_loop(const SearchArgs &a) {
         run_query<MODE, FILT, FAST, AUX>(a, L, V, qi, slot, M);
     }
     // Leave the bitset clean for the next launch.
-    if (V.dirty) {
-        for (int64_t w = lane_id(); w < a.bitset_words; w += 64) V.bits[w] = 0u;
-    }
+    if (V.dirty) visited_clear(a, V);
 }
 
 // General kernel (any ef, filters): LDS-bound, so no occupancy target for the compiler.
@@ -1095,6 +1125,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void se
     search_loop<MODE, false, true, AUX>(a);
 }
 
+// Register-beam kernel with the learned stop: the stop model's 18 values per lane do not fit
+// under 128 VGPRs beside the beam, so three waves per SIMD.
+template <int MODE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void search_stop_kernel(
+    SearchArgs a) {
+    search_loop<MODE, false, true, 2>(a);
+}
+
 __global__ void reset_queue_kernel(int32_t *q) { q[0] = 0; }
 
 template __global__ void search_kernel<kExactU8, false>(SearchArgs);
@@ -1107,8 +1145,8 @@ template __global__ void search_fast_kernel<kSeqF32, 0>(SearchArgs);
 template __global__ void search_fast_kernel<kSeqU8, 0>(SearchArgs);
 template __global__ void search_fast_kernel<kExactU8, 1>(SearchArgs);
 template __global__ void search_fast_kernel<kSeqF32, 1>(SearchArgs);
-template __global__ void search_fast_kernel<kExactU8, 2>(SearchArgs);
-template __global__ void search_fast_kernel<kSeqF32, 2>(SearchArgs);
+template __global__ void search_stop_kernel<kExactU8>(SearchArgs);
+template __global__ void search_stop_kernel<kSeqF32>(SearchArgs);
 
 }  // namespace hm
 
@@ -1120,7 +1158,7 @@ using KernelFn = void (*)(SearchArgs);
 
 static KernelFn pick(int mode, bool filt, bool fast, int aux) {
     if (aux == 1) return mode == kExactU8 ? search_fast_kernel<kExactU8, 1> : search_fast_kernel<kSeqF32, 1>;
-    if (aux == 2) return mode == kExactU8 ? search_fast_kernel<kExactU8, 2> : search_fast_kernel<kSeqF32, 2>;
+    if (aux == 2) return mode == kExactU8 ? search_stop_kernel<kExactU8> : search_stop_kernel<kSeqF32>;
     if (fast && !filt) {
         if (mode == kExactU8) return search_fast_kernel<kExactU8, 0>;
         if (mode == kSeqF32) return search_fast_kernel<kSeqF32, 0>;

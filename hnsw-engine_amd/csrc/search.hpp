@@ -52,6 +52,8 @@ struct SearchArgs {
     int32_t *queue;
     uint32_t *bitsets;
     int64_t bitset_words;
+    uint32_t *vlist;  // per wave slot: vlist_cap ids set in its bitset (clearing list)
+    int vlist_cap;
     uint64_t *spill_key;
     uint32_t *spill_id;
     int64_t spill_cap;
