@@ -826,3 +826,40 @@ extern "C" int hm_exact_timing(double *out) {
     out[3] = (double)g_timing.redo;
     return HM_OK;
 }
+
+extern "C" int hm_index_exact(hm_index *h, const float *queries, int64_t nq, int32_t k,
+                              const uint8_t *allowed, int64_t *ids, double *dists) {
+    if (!h || !queries || !ids || !dists) return invalid("null argument");
+    if (h->n == 0) return invalid("cannot search an empty index");
+    const int st = check_exact(h->n, h->dim, h->metric, k);
+    if (st != HM_OK) return st;
+    if (nq == 0) return HM_OK;
+    HM_TRY(hipSetDevice(h->device), "hipSetDevice");
+    const bool int8 = h->exact_u8 && h->metric != HM_COSINE && h->dim <= 1024 &&
+                      u8_values(queries, (size_t)nq * h->dim);
+    Scoped<float> dq;
+    Scoped<uint32_t> bits;
+    Scoped<int64_t> oid;
+    Scoped<double> od;
+    HM_TRY(dq.alloc((size_t)nq * h->dim), "alloc");
+    HM_TRY(oid.alloc((size_t)nq * k), "alloc");
+    HM_TRY(od.alloc((size_t)nq * k), "alloc");
+    HM_TRY(hipMemcpyAsync(dq.p, queries, (size_t)nq * h->dim * 4, hipMemcpyHostToDevice, h->stream),
+           "copy queries");
+    if (allowed) {
+        std::vector<uint32_t> b((size_t)(h->n + 31) / 32, 0u);
+        for (int64_t i = 0; i < h->n; ++i)
+            if (allowed[i]) b[i / 32] |= 1u << (i % 32);
+        HM_TRY(bits.alloc(b.size()), "alloc");
+        HM_TRY(hipMemcpyAsync(bits.p, b.data(), b.size() * 4, hipMemcpyHostToDevice, h->stream),
+               "copy mask");
+    }
+    const int rc = exact_device(h->vec.p, h->n, h->dim, h->metric, dq.p, nq, k,
+                                allowed ? bits.p : nullptr, int8, oid.p, od.p, h->stream, h->device);
+    if (rc != HM_OK) return rc;
+    HM_TRY(hipMemcpyAsync(ids, oid.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, h->stream), "copy ids");
+    HM_TRY(hipMemcpyAsync(dists, od.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, h->stream),
+           "copy dists");
+    HM_TRY(hipStreamSynchronize(h->stream), "exact search");
+    return HM_OK;
+}

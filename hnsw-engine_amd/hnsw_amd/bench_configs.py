@@ -1,0 +1,99 @@
+"""Measurements of the secondary workloads (the headline one is ``bench.py`` at the repository
+root). ``python -m hnsw_amd.bench_configs filtered`` prints one JSON line.
+
+``filtered``: 100,000 x 784 8-bit vectors (about half exact zeros, 10 prototypes plus uniform
+noise), 5,000 queries, k = 10, ef = 128, L2, M = 16, ef_construction = 200; masks of
+selectivity 1 %, 5 %, 10 % and 50 %, i.i.d. random and one contiguous id range. For each mask:
+the in-graph filtered search (time and tie-aware recall@10 against the exact top-10 over the
+allowed rows), the masked brute force (time), and the planner's choice after calibrating its
+threshold on this index. Times are host wall times of whole batches (queries uploaded, results
+downloaded), best of 3. (Parity of both paths with the C oracle at 784 dimensions is a test:
+tests/test_filtered784_gpu.py.)
+"""
+
+from __future__ import annotations
+
+import json
+import sys
+import time
+
+import numpy as np
+
+
+def _best(fn, repeats: int = 3):
+    out = fn()
+    best = float("inf")
+    for _ in range(repeats):
+        t = time.perf_counter()
+        out = fn()
+        best = min(best, time.perf_counter() - t)
+    return best, out
+
+
+def _exact_l2(data: np.ndarray, queries: np.ndarray, ids: np.ndarray) -> np.ndarray:
+    out = np.full(ids.shape, np.inf)
+    for i in range(ids.shape[0]):
+        v = ids[i] >= 0
+        diff = data[ids[i, v]].astype(np.int64) - queries[i].astype(np.int64)
+        out[i, v] = (diff * diff).sum(axis=1)
+    return out
+
+
+def filtered(log=print) -> dict:
+    from .planner import FilterPlanner
+    from .recall import recall_at_k
+    from .synthetic import random_mask, range_mask
+    from .workloads import WORKLOADS, load_or_build
+
+    w = WORKLOADS["sparse784_100k"]
+    eng, data, queries = load_or_build(w, log=log)
+    n, k, ef = data.shape[0], 10, 128
+    planner = FilterPlanner()
+    calib = planner.calibrate(eng, queries[:1000], k, ef,
+                              selectivities=(0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.5))
+    log(f"[filtered] planner thresholds: random {planner.random_threshold}, "
+        f"correlated {planner.correlated_threshold}; timings {calib}")
+    rows = []
+    for kind in ("random", "range"):
+        for i, s in enumerate((0.01, 0.05, 0.1, 0.5)):
+            mask = random_mask(n, s, 100 + i) if kind == "random" else range_mask(n, s, 200 + i)
+            kk = min(k, int(mask.sum()))
+            t_exact, (e_ids, e_d) = _best(lambda: eng.exact_search(queries, kk, mask))
+            t_graph, got = _best(lambda: eng.search_many(queries, k, ef, mask_pool=mask,
+                                                         mask_index=0))
+            g_ids = got[0]
+            rec = recall_at_k(g_ids[:, :kk], _exact_l2(data, queries, g_ids[:, :kk]), e_ids,
+                              e_d, kk)
+            plan = planner.plan(eng, mask)
+            row = {
+                "mask": kind, "selectivity": s, "allowed": int(mask.sum()),
+                "graph_qps": queries.shape[0] / t_graph, "graph_recall": rec["recall"],
+                "graph_dist_per_query": float(got[2].mean()),
+                "exact_qps": queries.shape[0] / t_exact,
+                "plan": plan.path, "plan_correlation": plan.correlation,
+            }
+            row["planned_qps"] = row["exact_qps"] if plan.path == "exact" else row["graph_qps"]
+            rows.append(row)
+            log(f"[filtered] {json.dumps(row)}")
+    return {
+        "workload": f"{w.name}: {n} x {w.dim}, {queries.shape[0]} queries, k={k}, ef={ef}",
+        "planner": {"random_threshold": planner.random_threshold,
+                    "correlated_threshold": planner.correlated_threshold,
+                    "calibration": [{"selectivity": s, "graph_s": g, "exact_s": e}
+                                    for s, g, e in calib]},
+        "rows": rows,
+    }
+
+
+def main(argv: list[str]) -> int:
+    which = argv[1] if len(argv) > 1 else "filtered"
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if which == "filtered":
+        print(json.dumps(filtered(log=log)), flush=True)
+        return 0
+    print(f"unknown workload {which!r}", file=sys.stderr)
+    return 2
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv))

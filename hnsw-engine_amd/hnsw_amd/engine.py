@@ -228,6 +228,19 @@ class MI355XEngine:
                                                       int(min_steps)))
         self._stop = (p, float(threshold), int(min_steps))
 
+    def exact_search(self, queries: np.ndarray, k: int, allowed: np.ndarray | None = None):
+        """Exact k-NN (ids, float64 distances) over the vectors in HBM, optionally restricted to
+        the allowed rows (the masked brute force, ``benchmarks/run_filtered.py:76-106``)."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), dtype=np.int64)
+        dists = np.empty((nq, k), dtype=np.float64)
+        mask = None if allowed is None else np.ascontiguousarray(allowed, dtype=np.uint8)
+        _native.check(_native.lib().hm_index_exact(self._device(), _native.ptr(q), nq, int(k),
+                                                   _native.ptr(mask), _native.ptr(ids),
+                                                   _native.ptr(dists)))
+        return ids, dists
+
     def trace(self, queries: np.ndarray, k: int, ef: int, thresholds: np.ndarray, steps: int = 128):
         """Plain register-beam search recording the stop features per layer-0 expansion.
 

@@ -217,6 +217,36 @@ class HNSWIndex:
             return np.full(shape, -1, dtype=np.int64), np.full(shape, np.inf)
         return self._engine.batch_search(prepared, k_valid, ef_valid, allowed)
 
+    def batch_search_planned(self, queries, k: int = 10, ef: int | None = None, filter=None,
+                             planner=None):
+        """Extension: :meth:`batch_search` with a filter planner (``planner.FilterPlanner``,
+        default thresholds when None). Returns ``(ids, dists, plan)``: the in-graph filtered
+        search (the reference's results) when ``plan.path == "graph"``, the exact top-k over
+        the allowed vectors when ``"exact"``. Without a filter this is :meth:`batch_search`."""
+        from .planner import FilterPlan, FilterPlanner
+
+        k_valid, ef_valid = self._check_search_args(k, ef)
+        allowed = self._check_filter(filter)
+        prepared = self._prepare(queries)
+        shape = (prepared.shape[0], k_valid)
+        if allowed is None:
+            ids, dists = self._engine.batch_search(prepared, k_valid, ef_valid, None)
+            return ids, dists, FilterPlan("graph", 1.0, 1.0, 0.0)
+        if not allowed.any():
+            return (np.full(shape, -1, dtype=np.int64), np.full(shape, np.inf),
+                    FilterPlan("exact", 0.0, 1.0, 0.0))
+        plan = (planner or FilterPlanner()).plan(self._engine, allowed)
+        if plan.path == "graph":
+            ids, dists = self._engine.batch_search(prepared, k_valid, ef_valid, allowed)
+            return ids, dists, plan
+        kk = min(k_valid, int(allowed.sum()))
+        got_ids, got_d = self._engine.exact_search(prepared, kk, allowed)
+        ids = np.full(shape, -1, dtype=np.int64)
+        dists = np.full(shape, np.inf)
+        ids[:, :kk] = got_ids
+        dists[:, :kk] = got_d
+        return ids, dists, plan
+
     def batch_search_mixed(self, queries, k, ef, filters=None, filter_index=None):
         """Extension: one batch with per-query ``k`` and ``ef`` (arrays) and per-query filters
         (``filters``: a pool of boolean rows; ``filter_index[i]`` picks query i's row, -1 for

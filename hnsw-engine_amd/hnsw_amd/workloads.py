@@ -23,7 +23,7 @@ from pathlib import Path
 import numpy as np
 
 from .engine import MI355XEngine
-from .synthetic import clustered_u8, clustered_u8_queries
+from .synthetic import clustered_u8, clustered_u8_queries, sparse_prototypes_u8
 
 CACHE = Path(__file__).resolve().parents[2] / ".graph_cache"
 
@@ -41,13 +41,19 @@ class Workload:
     M: int = 16
     ef_construction: int = 200
     metric: str = "l2"
+    kind: str = "clustered"  # or "sparse_prototypes" (n_clusters = prototypes)
 
     def data(self) -> tuple[np.ndarray, np.ndarray]:
+        if self.kind == "sparse_prototypes":
+            return sparse_prototypes_u8(self.n, self.n_queries, dim=self.dim,
+                                        n_prototypes=self.n_clusters, seed=self.seed)
         return clustered_u8(self.n, self.dim, self.n_queries, n_clusters=self.n_clusters,
                             sigma=self.sigma, seed=self.seed)
 
     def train_queries(self, count: int) -> np.ndarray:
         """Queries from the same clusters, disjoint from the measured ones (seed + 2)."""
+        if self.kind != "clustered":
+            raise ValueError(f"{self.name} has no training-query generator")
         return clustered_u8_queries(count, self.dim, n_clusters=self.n_clusters,
                                     sigma=self.sigma, seed=self.seed, query_seed=self.seed + 2)
 
@@ -61,6 +67,10 @@ WORKLOADS = {
     "clustered_1m": Workload("clustered_1m", 1_000_000, 128, 10_000, 1000, 30.0, 20260101, 7),
     "clustered_100k": Workload("clustered_100k", 100_000, 128, 1_000, 100, 30.0, 20260102, 7),
     "clustered_200k": Workload("clustered_200k", 200_000, 128, 2_000, 200, 30.0, 20260103, 7),
+    # Filtered search: 100,000 x 784 8-bit vectors, about half exact zeros, 10 prototypes plus
+    # uniform noise; 5,000 queries.
+    "sparse784_100k": Workload("sparse784_100k", 100_000, 784, 5_000, 10, 0.0, 20260104, 7,
+                               kind="sparse_prototypes"),
 }
 
 

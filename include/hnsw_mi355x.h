@@ -145,6 +145,13 @@ int hm_exact_dev(const float *d_data, int64_t n, int32_t dim, int32_t metric,
                  const float *d_queries, int64_t nq, int32_t k, const uint32_t *d_allowed_bits,
                  int exact_u8, int64_t *d_ids, double *d_dists, void *stream);
 
+/* The exact search over the vectors an index already keeps in HBM (the masked brute force of a
+ * filter planner; benchmarks/run_filtered.py:76-106 is the reference's exact_subset_search):
+ * queries float32 (nq, dim) host, allowed bool bytes (n,) host or null, outputs host int64 /
+ * float64 (nq, k). Same results as hm_exact on the index's vectors. */
+int hm_index_exact(hm_index *index, const float *queries, int64_t nq, int32_t k,
+                   const uint8_t *allowed, int64_t *ids, double *dists);
+
 /* Instrumentation (no counterpart in the reference): kernel times in ms of the last exact
  * search on the calling thread that took the int8 range-search path, out[0] bound pass plus
  * threshold, out[1] range pass, out[2] collect, and out[3] the number of queries it re-ran on
