@@ -173,6 +173,20 @@ def main() -> int:
                 lsweep.append(r)
                 log(f"[bench] learned ef={ef_l} threshold={tau} recall={r['recall']:.4f} "
                     f"qps={r['qps']:,.0f} dist/q={r['dcount_mean']:.0f} hops/q={r['hops_mean']:.0f}")
+        # Ceiling of any stop rule on this beam: the oracle stopping points of the measured
+        # queries (known future) at the target recall, against the plain beam's expansions.
+        thr_m = lrn.recall_thresholds(true_d, k)
+        tr_p, len_p, _ = eng.trace(queries, k, plain["ef"], thr_m, steps=256)
+        tr_o, len_o, _ = eng.trace(queries, k, max(int(v) for v in args.learned_ef.split(",")),
+                                   thr_m, steps=256)
+        ceiling = lrn.oracle_stop(tr_o, len_o, k, args.target_recall)
+        ceiling_info = None
+        if ceiling is not None:
+            ceiling_info = {"layer0_expansions": ceiling[0], "recall": ceiling[1],
+                           "plain_layer0_expansions": float(np.minimum(len_p, 256).mean()),
+                           "max_speedup_in_expansions":
+                               float(np.minimum(len_p, 256).mean()) / ceiling[0]}
+            log(f"[bench] stop ceiling: {ceiling_info}")
         ok = [r for r in lsweep if r["recall"] >= args.target_recall]
         best = max(ok, key=lambda r: r["qps"]) if ok else None
         learned_best = best
@@ -180,6 +194,7 @@ def main() -> int:
             "train_rows": int(x.shape[0]), "positive_share": float(y.mean()),
             "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
             "train_seconds": t_train,
+            "oracle_stop_ceiling": ceiling_info,
             "sweep": [{kk: r[kk] for kk in ("ef", "threshold", "recall", "qps", "dcount_mean",
                                              "hops_mean")} for r in lsweep],
         }

@@ -61,6 +61,38 @@ def dataset_from_traces(trace: np.ndarray, length: np.ndarray, final: np.ndarray
     return np.ascontiguousarray(x, dtype=np.float32), np.ascontiguousarray(y, dtype=np.float32)
 
 
+def oracle_stop(trace: np.ndarray, length: np.ndarray, k: int, target_recall: float):
+    """The cheapest per-query stopping points (in layer-0 expansions) that keep the mean
+    recall of the traced queries at ``target_recall``, knowing every query's future: for a
+    price ``lam`` per expansion each query stops where ``hits_t - lam t`` is largest, and the
+    price is bisected until the mean hits reach the target. Returns ``(mean expansions, mean
+    recall)``, or ``None`` when even the full searches stay below the target. A ceiling for
+    any stop rule on the same beam."""
+    steps = trace.shape[1]
+    ln = np.minimum(length, steps)
+    hits = trace[..., N_FEATURES].astype(np.float64)
+    t = np.arange(1, steps + 1, dtype=np.float64)[None, :]
+    valid = t <= ln[:, None]
+
+    def pick(lam: float):
+        score = np.where(valid, hits - lam * t, -np.inf)
+        j = np.argmax(score, axis=1)
+        rows = np.arange(trace.shape[0])
+        return float(t[0, j].mean()), float(hits[rows, j].mean() / k)
+
+    full = pick(0.0)
+    if full[1] < target_recall:
+        return None
+    lo, hi = 0.0, float(k)
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        if pick(mid)[1] >= target_recall:
+            lo = mid
+        else:
+            hi = mid
+    return pick(lo)
+
+
 def init_model(x: np.ndarray, seed: int = 0) -> np.ndarray:
     """Feature normalisation from ``x`` and He-initialised weights."""
     rng = np.random.default_rng(seed)

@@ -47,3 +47,20 @@ def test_labels_mark_steps_whose_hits_equal_the_final_hits() -> None:
     assert x.shape == (8, 16)
     np.testing.assert_array_equal(y, [0, 0, 0, 1, 1, 0, 1, 1])
     np.testing.assert_array_equal(x[:, 0], [0, 1, 2, 3, 4, 0, 1, 2])
+
+
+def test_oracle_stop_on_hand_traces() -> None:
+    """Two queries, k = 2: query 0 reaches both hits at step 1, query 1 its first hit at step 2
+    and the second at step 5. Full recall costs (1 + 5) / 2 = 3 expansions; recall 0.75 lets
+    query 1 stop at step 2: (1 + 2) / 2 = 1.5."""
+    from hnsw_amd.learned import N_FEATURES, oracle_stop
+
+    trace = np.zeros((2, 6, N_FEATURES + 2), dtype=np.float32)
+    trace[0, :, N_FEATURES] = [2, 2, 2, 2, 2, 2]
+    trace[1, :, N_FEATURES] = [0, 1, 1, 1, 2, 2]
+    length = np.array([6, 6])
+    exp_full, rec_full = oracle_stop(trace, length, 2, 1.0)
+    assert rec_full == 1.0 and exp_full == 3.0
+    exp, rec = oracle_stop(trace, length, 2, 0.75)
+    assert rec == 0.75 and exp == 1.5
+    assert oracle_stop(trace, np.array([1, 1]), 2, 0.8) is None
