@@ -1,5 +1,5 @@
 """Measurements of the secondary workloads (the headline one is ``bench.py`` at the repository
-root). ``python -m hnsw_amd.bench_configs filtered`` prints one JSON line.
+root). ``python -m hnsw_amd.bench_configs filtered|quantized|mixed`` prints one JSON line.
 
 ``filtered``: 100,000 x 784 8-bit vectors (about half exact zeros, 10 prototypes plus uniform
 noise), 5,000 queries, k = 10, ef = 128, L2, M = 16, ef_construction = 200; masks of
@@ -85,14 +85,77 @@ def filtered(log=print) -> dict:
     }
 
 
+def quantized(log=print) -> dict:
+    """The QuantizedIndex path on angular data: the graph searched with uint8-code distances
+    (float64 code-space queries, the reference's batch_search_codes), then the ef candidates
+    re-ranked by exact float64 distance (rerank = ef); recall against the exact top-10 (float64
+    brute force over the float32 vectors); ef from 32 to 512."""
+    from .exact import exact_distances
+    from .quantization import ScalarQuantizer
+    from .recall import recall_at_k
+    from .workloads import WORKLOADS, load_or_build
+
+    w = WORKLOADS["angular_400k"]
+    eng, data, queries = load_or_build(w, log=log)
+    k = 10
+    t_exact, (true_ids, true_d) = _best(lambda: eng.exact_search(queries, k), 1)
+    quant = ScalarQuantizer.fit(data)
+    codes = quant.encode(data)
+    qcode = quant.transform_queries(queries, w.metric)
+    rows = []
+    for ef in (32, 64, 128, 256, 512):
+        def run():
+            ids, _ = eng.batch_search_codes(codes, qcode, ef, ef)
+            return eng.rerank(queries, ids, k)
+        t, (ids, dists) = _best(run)
+        t_codes, _ = _best(lambda: eng.batch_search_codes(codes, qcode, ef, ef))
+        # scored as the reference scores recall: float64 distances with normalised rows
+        rec = recall_at_k(ids, exact_distances(data, queries, ids, w.metric), true_ids, true_d, k)
+        row = {"ef": ef, "rerank": ef, "qps": queries.shape[0] / t,
+               "codes_search_qps": queries.shape[0] / t_codes, "recall": rec["recall"]}
+        rows.append(row)
+        log(f"[quantized] {json.dumps(row)}")
+    return {"workload": f"{w.name}: {w.n} x {w.dim} cosine, {queries.shape[0]} queries, k={k}",
+            "quantizer": {"step": float(quant.step)}, "exact_seconds": t_exact, "rows": rows}
+
+
+def mixed(log=print) -> dict:
+    """One persistent launch over 100,000 mixed queries on the 1M x 128 index: ef drawn from
+    {16, 32, 64, 128, 256, 512}, k from {1, 10, 100}, half of the queries with an i.i.d. mask of
+    selectivity 5-50 % (a pool of 8 masks). Each query follows exactly the reference semantics
+    of its own arguments."""
+    from .synthetic import clustered_u8_queries, random_mask
+    from .workloads import WORKLOADS, load_or_build
+
+    w = WORKLOADS["clustered_1m"]
+    eng, data, _ = load_or_build(w, log=log)
+    nq = 100_000
+    queries = clustered_u8_queries(nq, w.dim, n_clusters=w.n_clusters, sigma=w.sigma,
+                                   seed=w.seed, query_seed=w.seed + 3)
+    rng = np.random.default_rng(5)
+    k = rng.choice([1, 10, 100], size=nq)
+    ef = rng.choice([16, 32, 64, 128, 256, 512], size=nq)
+    sels = np.linspace(0.05, 0.5, 8)
+    pool = np.stack([random_mask(w.n, s, 300 + i) for i, s in enumerate(sels)])
+    midx = np.where(rng.random(nq) < 0.5, rng.integers(0, len(sels), size=nq), -1)
+    t, got = _best(lambda: eng.search_many(queries, k, np.maximum(ef, k), mask_pool=pool,
+                                           mask_index=midx), 2)
+    plain = midx < 0
+    return {"workload": f"{w.name}: {nq} mixed queries (ef 16-512, k 1/10/100, "
+                        f"{int((~plain).sum())} filtered at 5-50 %)",
+            "qps": nq / t, "seconds": t,
+            "mean_distances": float(got[2].mean()), "mean_hops": float(got[3].mean())}
+
+
 def main(argv: list[str]) -> int:
     which = argv[1] if len(argv) > 1 else "filtered"
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-    if which == "filtered":
-        print(json.dumps(filtered(log=log)), flush=True)
-        return 0
-    print(f"unknown workload {which!r}", file=sys.stderr)
-    return 2
+    runs = {"filtered": filtered, "quantized": quantized, "mixed": mixed}
+    if which not in runs:
+        print(f"unknown workload {which!r}; one of {sorted(runs)}", file=sys.stderr)
+        return 2
+    print(json.dumps(runs[which](log=log)), flush=True)
+    return 0
 
 
 if __name__ == "__main__":

@@ -23,7 +23,13 @@ from pathlib import Path
 import numpy as np
 
 from .engine import MI355XEngine
-from .synthetic import clustered_u8, clustered_u8_queries, sparse_prototypes_u8
+from .metrics import prepare
+from .synthetic import (
+    angular_heavy_tailed,
+    clustered_u8,
+    clustered_u8_queries,
+    sparse_prototypes_u8,
+)
 
 CACHE = Path(__file__).resolve().parents[2] / ".graph_cache"
 
@@ -41,12 +47,16 @@ class Workload:
     M: int = 16
     ef_construction: int = 200
     metric: str = "l2"
-    kind: str = "clustered"  # or "sparse_prototypes" (n_clusters = prototypes)
+    kind: str = "clustered"  # or "sparse_prototypes" (n_clusters = prototypes), "angular"
 
     def data(self) -> tuple[np.ndarray, np.ndarray]:
         if self.kind == "sparse_prototypes":
             return sparse_prototypes_u8(self.n, self.n_queries, dim=self.dim,
                                         n_prototypes=self.n_clusters, seed=self.seed)
+        if self.kind == "angular":  # prepared as the facade prepares cosine vectors
+            data, queries = angular_heavy_tailed(self.n, self.dim, self.n_queries,
+                                                 n_clusters=self.n_clusters, seed=self.seed)
+            return prepare(data, self.metric), prepare(queries, self.metric)
         return clustered_u8(self.n, self.dim, self.n_queries, n_clusters=self.n_clusters,
                             sigma=self.sigma, seed=self.seed)
 
@@ -71,6 +81,10 @@ WORKLOADS = {
     # uniform noise; 5,000 queries.
     "sparse784_100k": Workload("sparse784_100k", 100_000, 784, 5_000, 10, 0.0, 20260104, 7,
                                kind="sparse_prototypes"),
+    # Quantized search: 400,000 x 100 L2-normalised vectors from 200 Gaussian clusters with
+    # heavy-tailed (log-normal) per-dimension scales, cosine; 10,000 queries.
+    "angular_400k": Workload("angular_400k", 400_000, 100, 10_000, 200, 0.0, 20260105, 7,
+                             metric="cosine", kind="angular"),
 }
 
 
