@@ -267,19 +267,10 @@ int hm_set_stop_model(hm_index *h, const float *params, float threshold, int32_t
         return HM_OK;
     }
     if (!(threshold > 0.f && threshold < 1.f)) return invalid("threshold must be in (0, 1)");
-    // bfloat16 weights, then the normalisation folded into the first layer (the search kernel
-    // evaluates b1' + sum_i w1'_i f_i with w1' = w1 * istd, b1' = b1 - sum_i w1'_i mean_i).
+    // bfloat16 weights (the search kernel keeps them packed in pairs and normalises the
+    // features with the float32 means and inverse standard deviations).
     std::vector<float> w(params, params + kModelSize);
     for (int i = kModelW1; i < kModelSize; ++i) w[i] = bf16_host(w[i]);
-    for (int j = 0; j < kHidden; ++j) {
-        float shift = 0.f;
-        for (int i = 0; i < kFeatures; ++i) {
-            float &wij = w[kModelW1 + i * kHidden + j];
-            wij *= params[kFeatures + i];
-            shift += wij * params[i];
-        }
-        w[kModelB1 + j] -= shift;
-    }
     HM_TRY(hipSetDevice(h->device), "hipSetDevice");
     HM_TRY(h->stop_w.ensure(kModelSize), "alloc");
     HM_TRY(hipMemcpy(h->stop_w.p, w.data(), kModelSize * 4, hipMemcpyHostToDevice), "copy model");

@@ -755,24 +755,37 @@ struct StopState {
 // The model as the search kernel holds it: the feature normalisation is folded into the first
 // layer at upload (w1' = w1 / std, b1' = b1 - sum w1' mean), so a lane keeps 18 values.
 struct StopModel {
-    float w1[kFeatures], b1, w2, b2;  // this lane's hidden unit
+    uint32_t w1[kFeatures / 2];  // this lane's hidden unit: W1 column as bfloat16 pairs
+    uint32_t b1w2;               // b1 (low half) and W2 (high half), bfloat16
+    float b2;
 };
+
+HM_DEV uint32_t bf16_bits(float x) {  // the value is already a bfloat16: keep its high half
+    return __float_as_uint(x) >> 16;
+}
 
 HM_DEV void load_model(const SearchArgs &a, StopModel &M) {
     const int lane = lane_id();
 #pragma unroll
-    for (int i = 0; i < kFeatures; ++i) M.w1[i] = a.stop_w[kModelW1 + i * kHidden + lane];
-    M.b1 = a.stop_w[kModelB1 + lane];
-    M.w2 = a.stop_w[kModelW2 + lane];
+    for (int i = 0; i < kFeatures; i += 2)
+        M.w1[i / 2] = bf16_bits(a.stop_w[kModelW1 + i * kHidden + lane]) |
+                      (bf16_bits(a.stop_w[kModelW1 + (i + 1) * kHidden + lane]) << 16);
+    M.b1w2 = bf16_bits(a.stop_w[kModelB1 + lane]) | (bf16_bits(a.stop_w[kModelW2 + lane]) << 16);
     M.b2 = a.stop_w[kModelB2];
 }
 
-// 16 -> 64 (ReLU) -> 1 with float32 accumulation; lane j owns hidden unit j.
-HM_DEV float model_logit(const StopModel &M, const float *f) {
-    float z = M.b1;
+// 16 -> 64 (ReLU) -> 1 with float32 accumulation on normalised features (mean and inverse
+// standard deviation from the model, uniform across lanes); lane j owns hidden unit j.
+HM_DEV float model_logit(const SearchArgs &a, const StopModel &M, const float *f) {
+    float z = __uint_as_float(M.b1w2 << 16);
 #pragma unroll
-    for (int i = 0; i < kFeatures; ++i) z += M.w1[i] * f[i];
-    float y = M.w2 * fmaxf(z, 0.f);
+    for (int i = 0; i < kFeatures; i += 2) {
+        const float x0 = (f[i] - a.stop_w[i]) * a.stop_w[kFeatures + i];
+        const float x1 = (f[i + 1] - a.stop_w[i + 1]) * a.stop_w[kFeatures + i + 1];
+        z += __uint_as_float(M.w1[i / 2] << 16) * x0;
+        z += __uint_as_float(M.w1[i / 2] & 0xffff0000u) * x1;
+    }
+    float y = __uint_as_float(M.b1w2 & 0xffff0000u) * fmaxf(z, 0.f);
     for (int m = 32; m >= 1; m >>= 1) y += __shfl_xor(y, m, 64);
     return y + M.b2;
 }
@@ -965,6 +978,9 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
             const uint64_t kth = readlane_u64(rk, kk - 1);
             S.since = kth == S.last_kth ? S.since + 1 : 0;
             S.last_kth = kth;
+            // The learned stop evaluates (and so computes features) every second expansion once
+            // stop_tmin expansions ran; traces record every expansion.
+            if (AUX == 2 && (S.t < a.stop_tmin || (S.t & 1) != 0)) continue;
             const uint64_t valid_now = rn >= 64 ? ~0ull : ((1ull << rn) - 1ull);
             const uint64_t live_now = valid_now & ~expd;
             uint64_t next_key = kInfKey;
@@ -992,9 +1008,8 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                     a.trace_len[qi] = S.t;
                     a.trace_final[qi] = hits;
                 }
-            } else if (S.t >= a.stop_tmin && (S.t & 1) == 0) {
-                // every other expansion: the model is evaluated as often as it pays
-                if (model_logit(M, f) > a.stop_logit) break;
+            } else if (model_logit(a, M, f) > a.stop_logit) {
+                break;
             }
         }
     }
@@ -1128,7 +1143,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void se
 // Register-beam kernel with the learned stop: the stop model's 18 values per lane do not fit
 // under 128 VGPRs beside the beam, so three waves per SIMD.
 template <int MODE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void search_stop_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void search_stop_kernel(
     SearchArgs a) {
     search_loop<MODE, false, true, 2>(a);
 }
