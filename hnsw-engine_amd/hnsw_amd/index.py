@@ -22,7 +22,9 @@ from .persistence import (
     GraphState,
     check_native_shapes,
     load_native,
+    load_state,
     save_native,
+    save_state,
     validate_native,
 )
 
@@ -347,6 +349,28 @@ class HNSWIndex:
         self._engine.hops = 0
 
     # -- persistence (the reference's native format) ---------------------------------------
+
+    def save(self, path: str | Path) -> None:
+        """The index (parameters, generator state, graph) as a ``.npz`` snapshot at ``path``
+        (the reference's ``HNSWIndex.save``, ``index.py:369-372``; same file format)."""
+        meta = {"params": self._params(), "rng_state": self._rng.bit_generator.state}
+        save_state(path, meta, self.graph_state())
+
+    @classmethod
+    def load(cls, path: str | Path, engine: str | None = None) -> "HNSWIndex":
+        """An index written by :meth:`save` (or by the reference), validated; ``engine``
+        defaults to ``"mi355x"`` (the reference's engine names are accepted and mapped to it)."""
+        meta, state = load_state(path)
+        params = dict(meta["params"])
+        params["engine"] = "mi355x" if engine is None else engine
+        index = cls(**params)
+        if state.vectors.shape[1:] != (index.dim,):
+            raise ValueError(
+                f"stored vectors have shape {state.vectors.shape}, expected dim {index.dim}"
+            )
+        index._rng.bit_generator.state = meta["rng_state"]
+        index._engine.load_state(state)
+        return index
 
     def save_native(self, directory: str | Path) -> None:
         """Write the native arrays and ``header.json`` (``index.py:390-403``)."""

@@ -1,4 +1,6 @@
-"""The reference's on-disk native format and its validation, plus CSR <-> native conversion.
+"""The reference's on-disk formats -- the ``.npz`` GraphState snapshot (``save_state`` /
+``load_state``) and the native ``.npy`` directory -- their validation, and CSR <-> native
+conversion.
 
 Format (``persistence.py:192-243`` of the reference): one ``<name>.npy`` per engine array and a
 ``header.json`` written last, holding ``format = "hnsw-engine-native"``, ``format_version = 1``,
@@ -81,6 +83,78 @@ def diff_states(a: GraphState, b: GraphState) -> list[str]:
         ):
             problems.append(f"layer {layer}: links differ")
     return problems
+
+
+STATE_FORMAT_VERSION = 1
+
+
+def validate_state(state: GraphState) -> None:
+    """``ValueError`` unless the snapshot is consistent and every link is in range and on its
+    layer (``persistence.py:124-152`` of the reference: the kernels do not bounds-check)."""
+    n = state.size
+    if state.vectors.ndim != 2 or state.vectors.shape[0] != n:
+        raise ValueError("vectors and levels disagree on the number of nodes")
+    if n == 0:
+        if state.entry_point != -1 or state.max_level != -1 or state.num_layers != 0:
+            raise ValueError("an empty graph has no entry point and no layers")
+        return
+    if (state.levels < 0).any():
+        raise ValueError("negative node level")
+    top = int(state.levels.max())
+    if state.max_level != top or state.num_layers != top + 1 or len(state.indices) != top + 1:
+        raise ValueError("the number of layers does not match the node levels")
+    if not 0 <= state.entry_point < n or int(state.levels[state.entry_point]) != top:
+        raise ValueError("the entry point is not a node of the top layer")
+    for layer in range(top + 1):
+        on_layer = int((state.levels >= layer).sum())
+        ptr, idx = state.indptr[layer], state.indices[layer]
+        if (ptr.shape != (on_layer + 1,) or ptr[0] != 0 or ptr[-1] != idx.shape[0]
+                or (np.diff(ptr) < 0).any()):
+            raise ValueError(f"layer {layer}: malformed link offsets")
+        if idx.size and ((idx < 0).any() or (idx >= n).any() or (state.levels[idx] < layer).any()):
+            raise ValueError(f"layer {layer}: a link points to a node outside the layer")
+
+
+def save_state(path: str | Path, meta: dict[str, Any], state: GraphState) -> None:
+    """``.npz`` snapshot at exactly ``path`` (the reference's ``save_state``,
+    ``persistence.py:155-169``): ``meta`` as a JSON string with ``format_version``, vectors,
+    levels, ``entry = [entry_point, max_level]``, then ``indptr_<l>`` / ``indices_<l>`` per
+    layer. Written without pickle."""
+    arrays: dict[str, np.ndarray] = {
+        "meta": np.array(json.dumps({"format_version": STATE_FORMAT_VERSION, **meta})),
+        "vectors": state.vectors,
+        "levels": state.levels,
+        "entry": np.array([state.entry_point, state.max_level], dtype=np.int64),
+    }
+    for layer in range(state.num_layers):
+        arrays[f"indptr_{layer}"] = state.indptr[layer]
+        arrays[f"indices_{layer}"] = state.indices[layer]
+    with open(path, "wb") as handle:
+        np.savez(handle, **arrays)
+
+
+def load_state(path: str | Path) -> tuple[dict[str, Any], GraphState]:
+    """``(meta, state)`` of a file written by :func:`save_state` (or the reference's), read
+    without pickle and validated."""
+    with np.load(path, allow_pickle=False) as data:
+        if "meta" not in data.files:
+            raise ValueError(f"{path} is not an hnsw-engine index file")
+        meta: dict[str, Any] = json.loads(str(data["meta"]))
+        version = meta.pop("format_version", None)
+        if version != STATE_FORMAT_VERSION:
+            raise ValueError(f"unsupported index format version {version!r}")
+        entry_point, max_level = (int(v) for v in data["entry"])
+        layers = max_level + 1
+        state = GraphState(
+            vectors=np.ascontiguousarray(data["vectors"], dtype=np.float32),
+            levels=data["levels"].astype(np.int64),
+            entry_point=entry_point,
+            max_level=max_level,
+            indptr=tuple(data[f"indptr_{i}"].astype(np.int64) for i in range(layers)),
+            indices=tuple(data[f"indices_{i}"].astype(np.int64) for i in range(layers)),
+        )
+    validate_state(state)
+    return meta, state
 
 
 def state_from_native(arrays: dict[str, np.ndarray], entry_point: int, max_level: int) -> GraphState:
