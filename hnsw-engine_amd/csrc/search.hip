@@ -88,8 +88,10 @@ HM_DEV Lds carve_fast(uint8_t *base, const SearchArgs &a, int mode) {
     L.ri[0] = (uint32_t *)p; p += 64 * 4;
     L.ci[0] = (uint32_t *)p; p += 64 * 4;
     L.nbal = (uint32_t *)p; p += 64 * 4;
-    L.nbi = (uint32_t *)p; p += 64 * 4;
-    L.nbacc = (int32_t *)p; p += 64 * 4;
+    // One expansion reads the new ids (nbi) and the partial sums (nbacc) before its merge
+    // stages R in rk[0] / ri[0] / nbal, so they share that space.
+    L.nbi = L.ri[0];
+    L.nbacc = (int32_t *)L.nbal;
     L.hash = (uint32_t *)p;
     return L;
 }
@@ -133,6 +135,39 @@ HM_DEV int32_t exact_partial(uint4 x, uint4 q, int metric) {
     return (int32_t)xx - 2 * (int32_t)xq;
 }
 
+// Partial sums of dist_exact for a power-of-two chunk count 2^LG per row (LG < 0: 2^lg at
+// run time): 64 >> lg rows per pass, each lane one 16-byte chunk, kBatch passes of loads in
+// flight; the group's first lane stores the row's sum into L.nbacc.
+template <int LG>
+HM_DEV void exact_rows(const SearchArgs &a, const Lds &L, int nnew, int lg_rt) {
+    const int lane = lane_id();
+    const int lg = LG >= 0 ? LG : lg_rt;
+    const int cpv = 1 << lg;
+    const int c = lane & (cpv - 1);
+    const uint4 q4 = ((const uint4 *)L.qb)[c];
+    const int vpi = 64 >> lg;
+    constexpr int kBatch = 4;
+    for (int v0 = 0; v0 < nnew; v0 += vpi * kBatch) {
+        uint4 xs[kBatch];
+#pragma unroll
+        for (int it = 0; it < kBatch; ++it) {
+            int v = v0 + it * vpi + (lane >> lg);
+            if (v0 + it * vpi < nnew && v < nnew) {
+                uint32_t id = L.nbi[v];
+                xs[it] = *(const uint4 *)(a.vec_u8 + (size_t)id * a.dimp + (size_t)c * 16);
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < kBatch; ++it) {
+            if (v0 + it * vpi >= nnew) break;
+            int v = v0 + it * vpi + (lane >> lg);
+            int32_t part = v < nnew ? exact_partial(xs[it], q4, a.metric) : 0;
+            part = group_sum(part, cpv);
+            if (c == 0 && v < nnew) L.nbacc[v] = part;
+        }
+    }
+}
+
 // Exact integer distances of the nnew ids in L.nbi to the 8-bit query. Lanes 0..nnew-1 get
 // their key. Every partial sum is an exact integer, so any summation order equals the
 // reference's float64 sum.
@@ -141,30 +176,7 @@ HM_DEV uint64_t dist_exact(const SearchArgs &a, const Lds &L, int nnew, int32_t 
     const int cpv = a.dimp >> 4;
     const uint4 *qv = (const uint4 *)L.qb;
     if ((cpv & (cpv - 1)) == 0 && cpv <= 64) {
-        const int lg = __builtin_ctz(cpv);
-        const int c = lane & (cpv - 1);
-        const uint4 q4 = qv[c];
-        const int vpi = 64 >> lg;
-        constexpr int kBatch = 8;
-        for (int v0 = 0; v0 < nnew; v0 += vpi * kBatch) {
-            uint4 xs[kBatch];
-#pragma unroll
-            for (int it = 0; it < kBatch; ++it) {
-                int v = v0 + it * vpi + (lane >> lg);
-                if (v0 + it * vpi < nnew && v < nnew) {
-                    uint32_t id = L.nbi[v];
-                    xs[it] = *(const uint4 *)(a.vec_u8 + (size_t)id * a.dimp + (size_t)c * 16);
-                }
-            }
-#pragma unroll
-            for (int it = 0; it < kBatch; ++it) {
-                if (v0 + it * vpi >= nnew) break;
-                int v = v0 + it * vpi + (lane >> lg);
-                int32_t part = v < nnew ? exact_partial(xs[it], q4, a.metric) : 0;
-                for (int m = 1; m < cpv; m <<= 1) part += shfl_xor(part, m);
-                if (c == 0 && v < nnew) L.nbacc[v] = part;
-            }
-        }
+        exact_rows<-1>(a, L, nnew, __builtin_ctz(cpv));
     } else {
         if (lane < nnew) L.nbacc[lane] = 0;
         wave_sync();
@@ -879,15 +891,22 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                 S.dc0 += nnew;
                 S.last_new += nnew;
             }
-            // accept_i <=> #{r in R: d_r <= d_i} + #{j < i new: d_j <= d_i} < ef
+            // accept_i <=> #{r in R: d_r <= d_i} + #{j < i new: d_j <= d_i} < ef. Once R is
+            // full, a neighbour at or above its worst key has ef entries of R at or below it
+            // (rejected) and is above every neighbour below that key (counts for none of
+            // them), so only the lanes strictly below the worst key take part.
+            const uint64_t all_new = nnew >= 64 ? ~0ull : ((1ull << nnew) - 1ull);
+            const uint64_t worst = readlane_u64(rk, ef - 1);
+            const uint64_t part = rn == ef ? ballot(lane < nnew && ki < worst) : all_new;
             int c_r = 0, c_n = 0;
-            for (int q = 0; q < nnew; ++q) {
+            for (uint64_t m = part; m; m &= m - 1ull) {
+                const int q = __ffsll((long long)m) - 1;
                 const uint64_t kq = readlane_u64(ki, q);
                 const int cnt = popc(ballot(lane < rn && rk <= kq));
                 if (lane == q) c_r = cnt;
                 if (q < lane && kq <= ki) c_n += 1;
             }
-            const bool acc = lane < nnew && c_r + c_n < ef;
+            const bool acc = ((part >> lane) & 1ull) && c_r + c_n < ef;
             const uint64_t am = ballot(acc);
             if (am == 0) continue;
             const int na = popc(am);

@@ -90,12 +90,13 @@ constexpr int kModelB2 = kModelW2 + kHidden;
 constexpr int kModelSize = kModelB2 + 1;
 constexpr int kTraceCols = kFeatures + 2;  // features, hits, |R|
 
-// Bytes of LDS one wave of the register-beam kernel needs: query, 64-entry R scatter buffer,
-// flags, 64-entry tie list, the new-neighbour staging and the visited hash.
+// Bytes of LDS one wave of the register-beam kernel needs: query, 64-entry R scatter buffer
+// with its flags (shared with the new-neighbour staging), 64-entry tie list and the visited
+// hash. 128 dimensions with the 2,048-slot hash: 10,112 bytes, 16 waves per CU.
 inline int lds_bytes_fast(int mode, int dim, int dimp, int h_log2) {
     auto up = [](int v) { return (v + 15) & ~15; };
     int q = mode == kExactU8 ? up(dimp) : up(dim * 8);
-    return q + 2 * (64 * 8 + 64 * 4) + 64 * 4 + 64 * 4 + 64 * 4 + (4 << h_log2);
+    return q + 2 * (64 * 8 + 64 * 4) + 64 * 4 + (4 << h_log2);
 }
 
 // Bytes of LDS one wave needs for the given capacities.

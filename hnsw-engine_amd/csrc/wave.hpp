@@ -47,6 +47,28 @@ HM_DEV uint64_t shfl_u64(uint64_t v, int src) {
 }
 HM_DEV int shfl_xor(int v, int m) { return __shfl_xor(v, m, kWave); }
 
+// Sum over aligned groups of g lanes (g a wave-uniform power of two, 1..64); every lane of a
+// group receives its group's sum. Inside a row of 16 lanes the exchanges are DPP operands
+// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror: after each step all
+// lanes of the current group hold equal values, so a mirror reads the other half's sum), taken
+// or not by a select instead of a branch; the 32- and 64-lane steps cross rows through
+// ds_bpermute. All lanes must be active.
+HM_DEV int group_sum(int v, int g) {
+    int t = __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);
+    v += g >= 2 ? t : 0;
+    t = __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);
+    v += g >= 4 ? t : 0;
+    t = __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);
+    v += g >= 8 ? t : 0;
+    t = __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false);
+    v += g >= 16 ? t : 0;
+    if (g >= 32) {
+        v += shfl_xor(v, 16);
+        if (g >= 64) v += shfl_xor(v, 32);
+    }
+    return v;
+}
+
 HM_DEV int wave_sum(int v) {
     for (int m = 32; m >= 1; m >>= 1) v += shfl_xor(v, m);
     return v;
