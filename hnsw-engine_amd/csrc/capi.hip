@@ -14,7 +14,7 @@
 
 namespace hm {
 
-int search_slots(int mode, bool filt, bool fast, int aux, int lds, int device_cus);
+int search_slots(int mode, bool filt, const SearchArgs &a, int lds, int device_cus);
 hipError_t search_launch(int mode, bool filt, const SearchArgs &a, int blocks, hipStream_t s);
 
 static thread_local std::string g_error;
@@ -68,13 +68,14 @@ void plan(SearchArgs &a, int mode, const hm_index *h, int max_ef, bool fast) {
     max_ef = std::max(1, max_ef);
     a.fast = fast ? 1 : 0;
     if (fast) {
-        // 2048 slots (8 KB): measured on the 1M x 128 workload for ef 32..64 it beats 4096
-        // slots (fewer waves fit) although ~6 % of queries at ef 48 outgrow it and continue on
-        // the global bitset; 1024 slots overflow for nearly every query.
+        // 2048 slots (4 KB of 16-bit slots for up to 2M ids): measured on the 1M x 128
+        // workload for ef 32..64 it beats 4096 slots (fewer waves fit) although ~6 % of queries
+        // at ef 48 outgrow it and continue on the global bitset; 1024 slots overflow for nearly
+        // every query.
         a.h_log2 = h->tune_fast_hlog ? h->tune_fast_hlog : (max_ef <= 8 ? 10 : 11);
         a.r_cap = 64;
         a.c_cap = 64;
-        a.lds_per_wave = lds_bytes_fast(mode, h->dim, h->dimp, a.h_log2);
+        a.lds_per_wave = lds_bytes_fast(mode, h->dim, h->dimp, a.h_log2, a.id_bits);
         return;
     }
     a.r_cap = (int)round_up(max_ef, 8);
@@ -92,6 +93,8 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, 
     a.links_up = h->links_up.p;
     a.upper_off = h->upper.p;
     a.n = h->n;
+    a.id_bits = std::max(1, ceil_log2(h->n));
+    a.probe_limit = h->tune_probe_limit;
     a.w0 = h->w0;
     a.w1 = h->w1;
     a.entry = (int)h->entry;
@@ -134,8 +137,7 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, 
         a.stop_tmin = h->stop_tmin;
     }
     if (a.lds_per_wave > 160 * 1024) return invalid("ef too large for the on-chip search state");
-    const int aux = a.trace ? 1 : (a.stop_w ? 2 : 0);
-    int slots = search_slots(mode, filt, a.fast != 0, aux, a.lds_per_wave, h->cus);
+    int slots = search_slots(mode, filt, a, a.lds_per_wave, h->cus);
     if (h->tune_waves_per_cu > 0) slots = std::min(slots, h->tune_waves_per_cu * h->cus);
     HM_TRY(h->stats.ensure(2, true, s), "stats allocation");
     a.stats = h->stats.p;
@@ -556,14 +558,16 @@ int hm_search_codes(hm_index *h, const double *queries, const hm_search_opts *o,
     return search_host(h, kSeqU8, nullptr, queries, o, ids, dists, dcount, hops);
 }
 
-int hm_set_tuning(hm_index *h, int32_t fast_hash_log2, int32_t hash_log2, int32_t waves_per_cu) {
+int hm_set_tuning(hm_index *h, int32_t fast_hash_log2, int32_t hash_log2, int32_t waves_per_cu,
+                  int32_t probe_limit) {
     if (!h) return invalid("null argument");
     if ((fast_hash_log2 && (fast_hash_log2 < 6 || fast_hash_log2 > 14)) ||
-        (hash_log2 && (hash_log2 < 6 || hash_log2 > 14)) || waves_per_cu < 0)
+        (hash_log2 && (hash_log2 < 6 || hash_log2 > 14)) || waves_per_cu < 0 || probe_limit < 0)
         return invalid("tuning values out of range");
     h->tune_fast_hlog = fast_hash_log2;
     h->tune_hlog = hash_log2;
     h->tune_waves_per_cu = waves_per_cu;
+    h->tune_probe_limit = probe_limit;
     return HM_OK;
 }
 

@@ -36,7 +36,7 @@ struct hm_index {
     bool stop_on = false;
     // tuning (0: automatic): visited-hash size of the register-beam and general kernels, waves
     // per compute unit
-    int tune_fast_hlog = 0, tune_hlog = 0, tune_waves_per_cu = 0;
+    int tune_fast_hlog = 0, tune_hlog = 0, tune_waves_per_cu = 0, tune_probe_limit = 0;
     hm::DevBuf<int32_t> stats;  // event counters, see SearchArgs::stats
 
     void release_all() {

@@ -96,6 +96,19 @@ HM_DEV Lds carve_fast(uint8_t *base, const SearchArgs &a, int mode) {
     return L;
 }
 
+// Compile-time shape of the fast kernels: kShapeAny reads the row widths from the arguments;
+// kShapeM16D128 is the common build (M = 16: 32 links per layer-0 row, 16 per upper row; 128
+// one-byte dimensions), where every loop bound and stride is a constant.
+constexpr int kShapeAny = 0;
+constexpr int kShapeM16D128 = 1;
+
+template <int SH> HM_DEV int w0_of(const SearchArgs &a) { return SH == kShapeM16D128 ? 32 : a.w0; }
+template <int SH> HM_DEV int w1_of(const SearchArgs &a) { return SH == kShapeM16D128 ? 16 : a.w1; }
+
+inline bool shape_m16d128(const SearchArgs &a, int mode) {
+    return mode == kExactU8 && a.w0 == 32 && a.w1 == 16 && a.dimp == 128;
+}
+
 // Number of entries of sorted (k, i)[0..n) strictly below (bk, bi).
 HM_DEV int lower_count(const uint64_t *k, const uint32_t *id, int n, uint64_t bk, uint32_t bi) {
     int lo = 0, hi = n;
@@ -154,7 +167,7 @@ HM_DEV void exact_rows(const SearchArgs &a, const Lds &L, int nnew, int lg_rt) {
             int v = v0 + it * vpi + (lane >> lg);
             if (v0 + it * vpi < nnew && v < nnew) {
                 uint32_t id = L.nbi[v];
-                xs[it] = *(const uint4 *)(a.vec_u8 + (size_t)id * a.dimp + (size_t)c * 16);
+                xs[it] = *(const uint4 *)(a.vec_u8 + ((size_t)id << (lg + 4)) + (size_t)c * 16);
             }
         }
 #pragma unroll
@@ -171,11 +184,14 @@ HM_DEV void exact_rows(const SearchArgs &a, const Lds &L, int nnew, int lg_rt) {
 // Exact integer distances of the nnew ids in L.nbi to the 8-bit query. Lanes 0..nnew-1 get
 // their key. Every partial sum is an exact integer, so any summation order equals the
 // reference's float64 sum.
+template <int SH>
 HM_DEV uint64_t dist_exact(const SearchArgs &a, const Lds &L, int nnew, int32_t qq) {
     const int lane = lane_id();
     const int cpv = a.dimp >> 4;
     const uint4 *qv = (const uint4 *)L.qb;
-    if ((cpv & (cpv - 1)) == 0 && cpv <= 64) {
+    if constexpr (SH == kShapeM16D128) {
+        exact_rows<3>(a, L, nnew, 3);
+    } else if ((cpv & (cpv - 1)) == 0 && cpv <= 64) {
         exact_rows<-1>(a, L, nnew, __builtin_ctz(cpv));
     } else {
         if (lane < nnew) L.nbacc[lane] = 0;
@@ -262,9 +278,9 @@ HM_DEV uint64_t dist_seq(const SearchArgs &a, const Lds &L, int nnew) {
     return key_of_double(d);
 }
 
-template <int MODE>
+template <int MODE, int SH = kShapeAny>
 HM_DEV uint64_t distances(const SearchArgs &a, const Lds &L, int nnew, int32_t qq) {
-    if constexpr (MODE == kExactU8) return dist_exact(a, L, nnew, qq);
+    if constexpr (MODE == kExactU8) return dist_exact<SH>(a, L, nnew, qq);
     else return dist_seq<MODE == kSeqU8>(a, L, nnew);
 }
 
@@ -276,15 +292,41 @@ HM_DEV double decode(uint64_t k, int metric) {
 
 // ---------------------------------------------------------------------------- visited set
 
+// Visited set of one layer search: an open-addressing hash in LDS (linear probing) until it is
+// half full, then a per-wave global bitset. Whole-id mode: one 32-bit slot per id (id + 1).
+// Quotient mode (register-beam kernel, when the ids have few bits more than the table index):
+// 16-bit slots, two per word. The id is first mapped by a bijection of [0, 2^B) (multiplication
+// by an odd constant modulo 2^B, B = max(id bits, hlog)); the top hlog bits of the image are the
+// home slot, the low r = B - hlog bits (the remainder) are stored with the probe distance from
+// home: slot = (remainder << (16 - r)) | (distance + 1). A slot therefore determines its id
+// exactly, and two entries are equal iff their ids are. A distance beyond the field moves the
+// set to the bitset like a full table does.
 struct Visited {
     int32_t *stats;
     uint32_t *bits;
     uint32_t *list;  // ids set in the bitset during this layer search (for clearing)
     int list_cap, list_n;  // list_n > list_cap: too many, clear the whole bitset
-    int hlog;    // active hash size is 1 << hlog
+    int hlog;    // active hash size is 1 << hlog slots
     int count;   // ids inserted in this layer
+    int id_bits;  // ids are < 2^id_bits
+    int rbits;    // quotient mode: remainder bits
+    int probe_limit;  // > 0: quotient mode moves to the bitset beyond this probe distance
     bool use_bits, dirty;
+    bool quotient_ok;  // this kernel's LDS holds 16-bit slots
+    bool quotient;     // the current layer search uses them
 };
+
+constexpr uint32_t kHashMul = 2654435761u;
+constexpr uint32_t inverse_odd(uint32_t a) {  // Newton iteration: 3, 6, 12, 24, 48 bits
+    uint32_t x = a;
+    for (int i = 0; i < 5; ++i) x *= 2u - a * x;
+    return x;
+}
+constexpr uint32_t kHashMulInv = inverse_odd(kHashMul);
+static_assert(kHashMul * kHashMulInv == 1u, "kHashMul must be invertible modulo 2^32");
+
+// Quotient mode is used when the remainder leaves at least 6 bits of probe distance.
+constexpr int kQuotientMaxRemainder = 10;
 
 // Zeroes the bitset words this layer search set: through the id list when it held them all,
 // otherwise the whole bitset.
@@ -318,24 +360,44 @@ HM_DEV void visited_reset(const SearchArgs &a, const Lds &L, Visited &V, int hlo
     V.use_bits = false;
     V.hlog = hlog;
     V.count = 0;
-    for (int i = lane; i < (1 << hlog); i += 64) L.hash[i] = 0u;
+    V.rbits = V.id_bits > hlog ? V.id_bits - hlog : 0;
+    V.quotient = V.quotient_ok && V.rbits <= kQuotientMaxRemainder;
+    const int words = V.quotient ? (1 << hlog) / 2 : (1 << hlog);
+    for (int i = lane; i < words; i += 64) L.hash[i] = 0u;
     wave_sync();
 }
 
-// Moves every hashed id into the bitset once the hash is half full.
-HM_DEV void visited_make_room(const Lds &L, Visited &V, int incoming) {
-    if (V.use_bits || V.count + incoming <= (1 << V.hlog) / 2) return;
+// Moves every hashed id into the bitset (called by the whole wave).
+HM_DEV void visited_to_bits(const Lds &L, Visited &V) {
     const int lane = lane_id();
     V.use_bits = true;
     V.dirty = true;
     V.list_n = 0;
+    const uint32_t hmask = (1u << V.hlog) - 1u;
+    const int r = V.rbits, db = 16 - r;
     for (int i0 = 0; i0 < (1 << V.hlog); i0 += 64) {
-        const uint32_t v = L.hash[i0 + lane];
+        const uint32_t slot = (uint32_t)(i0 + lane);
+        uint32_t v;  // id + 1, 0 for an empty slot
+        if (V.quotient) {
+            const uint32_t t = (L.hash[slot >> 1] >> ((slot & 1u) << 4)) & 0xFFFFu;
+            const uint32_t home = (slot - ((t & ((1u << db) - 1u)) - 1u)) & hmask;
+            const uint32_t h = (home << r) | (t >> db);
+            const uint32_t bmask = (uint32_t)((1ull << (V.hlog + r)) - 1ull);
+            v = t ? ((h * kHashMulInv) & bmask) + 1u : 0u;
+        } else {
+            v = L.hash[slot];
+        }
         if (v) atomicOr(&V.bits[(v - 1) >> 5], 1u << ((v - 1) & 31));
         visited_note(V, v != 0u, v - 1u);
     }
     __threadfence();
     if (lane == 0 && V.stats) atomicAdd(&V.stats[0], 1);
+}
+
+// Moves the set to the bitset once the hash would be more than half full.
+HM_DEV void visited_make_room(const Lds &L, Visited &V, int incoming) {
+    if (V.use_bits || V.count + incoming <= (1 << V.hlog) / 2) return;
+    visited_to_bits(L, V);
 }
 
 // True if `id` was not yet visited in this layer; marks it.
@@ -352,6 +414,56 @@ HM_DEV bool visit(const Lds &L, const Visited &V, uint32_t id) {
         if (old == v) return false;
         pos = (pos + 1u) & mask;
     }
+}
+
+// Quotient-mode visit of the active lanes' ids (distinct within one call); called by every lane.
+// A lane walks from its home slot comparing each slot with its own entry at that distance; it
+// claims the first empty slot by a CAS on the word holding it (the word is guessed empty first:
+// a failed CAS returns its contents, and a stale guess can only show fewer claimed slots than
+// there are, since a claimed slot never changes). A lane that runs out of distance bits moves
+// the whole set to the bitset (this call's insertions included) and retries there. Returns
+// true for a newly inserted id.
+HM_DEV bool visit_quotient(const Lds &L, Visited &V, uint32_t id, bool active) {
+    const int r = V.rbits, db = 16 - r;
+    const uint32_t hmask = (1u << V.hlog) - 1u;
+    const uint32_t bmask = (uint32_t)((1ull << (V.hlog + r)) - 1ull);
+    const uint32_t h = (id * kHashMul) & bmask;
+    const uint32_t home = h >> r, rem = h & ((1u << r) - 1u);
+    uint32_t maxd = (1u << db) - 2u;
+    if (V.probe_limit > 0 && (uint32_t)V.probe_limit < maxd) maxd = (uint32_t)V.probe_limit;
+    bool isnew = false, full = false;
+    if (active) {
+        uint32_t dist = 0u, w = 0u;  // w: contents (or guess) of the word holding the slot
+        int state;                   // 0 probing, 1 inserted, 2 present, 3 out of distance bits
+        do {  // branch-free step, one LDS operation: claim a slot that looks empty, else read
+            const uint32_t slot = (home + dist) & hmask;
+            const uint32_t sh = (slot & 1u) << 4;
+            const uint32_t mine = (rem << db) | (dist + 1u);
+            const uint32_t cur = (w >> sh) & 0xFFFFu;
+            const uint32_t old = atomicCAS(&L.hash[slot >> 1], w, cur == 0u ? w | (mine << sh) : w);
+            const bool claimed = old == w && cur == 0u;
+            const uint32_t now = (old >> sh) & 0xFFFFu;
+            const bool other = !claimed && now != mine && now != 0u;
+            state = claimed ? 1 : (now == mine ? 2 : (other && dist == maxd ? 3 : 0));
+            const bool step = other && dist != maxd;
+            dist += step ? 1u : 0u;
+            w = step && sh != 0u ? 0u : old;  // the slot after an odd one starts a new word
+        } while (state == 0);
+        isnew = state == 1;
+        full = state == 3;
+    }
+    if (ballot(full)) {
+        visited_to_bits(L, V);
+        if (full) isnew = visit(L, V, id);
+    }
+    return isnew;
+}
+
+// True for the active lanes whose id was not yet visited in this layer; marks them. Called by
+// every lane of the wave.
+HM_DEV bool visit_lanes(const Lds &L, Visited &V, uint32_t id, bool active) {
+    if (!V.quotient || V.use_bits) return active && visit(L, V, id);
+    return visit_quotient(L, V, id, active);
 }
 
 // ---------------------------------------------------------------------------- query state
@@ -371,23 +483,24 @@ HM_DEV bool allowed(const Query &Q, uint32_t id) {
 // the single best node; a neighbour is accepted iff strictly closer than the current best, so
 // the candidate after an expansion is the first occurrence of the row's minimum when that beats
 // the current best, and the search ends after the first expansion that does not improve.
-template <int MODE>
+template <int MODE, int SH>
 HM_DEV void greedy_layer(const SearchArgs &a, const Lds &L, Visited &V, Query &Q, int layer,
                          uint64_t &cur_k, uint32_t &cur_id) {
     const int lane = lane_id();
     visited_reset(a, L, V, a.h_log2 < 9 ? a.h_log2 : 9);
-    if (lane == 0) visit(L, V, cur_id);
+    visit_lanes(L, V, cur_id, lane == 0);
     V.count = 1;
     wave_sync();
     for (;;) {
         Q.hops += 1;
         const int64_t row = (int64_t)a.upper_off[cur_id] + layer - 1;
         bool improved = false;
-        for (int base = 0; base < a.w1; base += 64) {
+        const int w1 = w1_of<SH>(a);
+        for (int base = 0; base < w1; base += 64) {
             const int j = base + lane;
-            const int32_t nb = j < a.w1 ? a.links_up[row * a.w1 + j] : -1;
+            const int32_t nb = j < w1 ? a.links_up[row * w1 + j] : -1;
             visited_make_room(L, V, 64);
-            const bool isnew = nb >= 0 && visit(L, V, (uint32_t)nb);
+            const bool isnew = visit_lanes(L, V, (uint32_t)nb, nb >= 0);
             visited_note(V, isnew, (uint32_t)nb);
             const uint64_t m = ballot(isnew);
             const int nnew = popc(m);
@@ -395,7 +508,7 @@ HM_DEV void greedy_layer(const SearchArgs &a, const Lds &L, Visited &V, Query &Q
             if (isnew) L.nbi[rank_below(m)] = (uint32_t)nb;
             wave_sync();
             V.count += nnew;
-            uint64_t key = distances<MODE>(a, L, nnew, Q.qq);
+            uint64_t key = distances<MODE, SH>(a, L, nnew, Q.qq);
             Q.dcount += nnew;
             uint32_t idx = lane < nnew ? (uint32_t)lane : kInfId;
             wave_min_pair(key, idx);
@@ -485,7 +598,7 @@ HM_DEV bool layer0(const SearchArgs &a, const Lds &L, Visited &V, Query &Q, int 
     uint32_t *const si = a.spill_id + (size_t)slot * a.spill_cap;
 
     visited_reset(a, L, V, a.h_log2);
-    if (lane == 0) visit(L, V, entry_id);
+    visit_lanes(L, V, entry_id, lane == 0);
     V.count = 1;
 
     rb = 0;
@@ -586,7 +699,7 @@ HM_DEV bool layer0(const SearchArgs &a, const Lds &L, Visited &V, Query &Q, int 
             const int j = base + lane;
             const int32_t nb = j < a.w0 ? a.links0[(size_t)cid * a.w0 + j] : -1;
             visited_make_room(L, V, 64);
-            const bool isnew = nb >= 0 && visit(L, V, (uint32_t)nb);
+            const bool isnew = visit_lanes(L, V, (uint32_t)nb, nb >= 0);
             visited_note(V, isnew, (uint32_t)nb);
             const uint64_t nm = ballot(isnew);
             const int nnew = popc(nm);
@@ -831,7 +944,7 @@ HM_DEV void features(const SearchArgs &a, const Query &Q, const StopState &S, ui
 }
 
 // AUX: 0 plain, 1 trace (plain search that records features and hits), 2 learned stop.
-template <int MODE, int AUX>
+template <int MODE, int AUX, int SH>
 HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                         uint64_t entry_k, uint32_t entry_id, uint64_t &rk, uint32_t &ri,
                         int &rn, int qi, const StopModel &M) {
@@ -842,7 +955,7 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
     uint64_t *const tk = L.ck[0];  // evicted ties, sorted by id, [th, tn)
     uint32_t *const ti = L.ci[0];
     visited_reset(a, L, V, a.h_log2);
-    if (lane == 0) visit(L, V, entry_id);
+    visit_lanes(L, V, entry_id, lane == 0);
     V.count = 1;
     wave_sync();
     rk = lane == 0 ? entry_k : kInfKey;
@@ -872,11 +985,12 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
             S.last_new = 0;
             S.last_acc = 0;
         }
-        for (int base = 0; base < a.w0; base += 64) {
+        const int w0 = w0_of<SH>(a);
+        for (int base = 0; base < w0; base += 64) {
             const int j = base + lane;
-            const int32_t nb = j < a.w0 ? a.links0[(size_t)cid * a.w0 + j] : -1;
-            visited_make_room(L, V, min(64, a.w0 - base));
-            const bool isnew = nb >= 0 && visit(L, V, (uint32_t)nb);
+            const int32_t nb = j < w0 ? a.links0[(size_t)cid * w0 + j] : -1;
+            visited_make_room(L, V, min(64, w0 - base));
+            const bool isnew = visit_lanes(L, V, (uint32_t)nb, nb >= 0);
             visited_note(V, isnew, (uint32_t)nb);
             const uint64_t nm = ballot(isnew);
             const int nnew = popc(nm);
@@ -884,7 +998,7 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
             if (isnew) L.nbi[rank_below(nm)] = (uint32_t)nb;
             wave_sync();
             V.count += nnew;
-            const uint64_t ki = distances<MODE>(a, L, nnew, Q.qq);
+            const uint64_t ki = distances<MODE, SH>(a, L, nnew, Q.qq);
             const uint32_t ii = lane < nnew ? L.nbi[lane] : kInfId;
             Q.dcount += nnew;
             if constexpr (AUX != 0) {
@@ -1037,7 +1151,7 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
 
 // ---------------------------------------------------------------------------- one query
 
-template <int MODE, bool FILT, bool FAST, int AUX>
+template <int MODE, bool FILT, bool FAST, int AUX, int SH>
 HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int slot,
                       const StopModel &M) {
     const int lane = lane_id();
@@ -1073,12 +1187,12 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
     wave_sync();
 
     // Entry point distance (numba_engine.py:449-451), then greedy descent.
-    uint64_t cur_k = shfl_u64(distances<MODE>(a, L, 1, Q.qq), 0);
+    uint64_t cur_k = shfl_u64(distances<MODE, SH>(a, L, 1, Q.qq), 0);
     uint32_t cur_id = (uint32_t)a.entry;
     Q.dcount = 1;
     wave_sync();
     for (int layer = a.max_level; layer > 0; --layer)
-        greedy_layer<MODE>(a, L, V, Q, layer, cur_k, cur_id);
+        greedy_layer<MODE, SH>(a, L, V, Q, layer, cur_k, cur_id);
 
     int64_t *oi = a.out_ids + (size_t)qi * a.kstride;
     double *od = a.out_d + (size_t)qi * a.kstride;
@@ -1087,7 +1201,7 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
     if constexpr (FAST) {
         uint64_t rk;
         uint32_t ri;
-        ok = layer0_fast<MODE, AUX>(a, L, V, Q, cur_k, cur_id, rk, ri, rn, qi, M);
+        ok = layer0_fast<MODE, AUX, SH>(a, L, V, Q, cur_k, cur_id, rk, ri, rn, qi, M);
         if (!ok) rn = 0;
         const int take = min(Q.k, rn);
         for (int i = lane; i < a.kstride; i += 64) {
@@ -1118,7 +1232,7 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
     wave_sync();
 }
 
-template <int MODE, bool FILT, bool FAST, int AUX>
+template <int MODE, bool FILT, bool FAST, int AUX, int SH>
 HM_DEV void search_loop(const SearchArgs &a) {
     extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
     const int slot = blockIdx.x;
@@ -1126,6 +1240,11 @@ HM_DEV void search_loop(const SearchArgs &a) {
     Visited V;
     V.bits = a.bitsets + (size_t)slot * a.bitset_words;
     V.list = a.vlist + (size_t)slot * a.vlist_cap;
+    V.quotient_ok = FAST;
+    V.quotient = false;
+    V.id_bits = a.id_bits;
+    V.rbits = 0;
+    V.probe_limit = a.probe_limit;
     V.list_cap = a.vlist_cap;
     V.list_n = 0;
     V.stats = a.stats;
@@ -1140,7 +1259,7 @@ HM_DEV void search_loop(const SearchArgs &a) {
         if (lane_id() == 0) qi = atomicAdd(a.queue, 1);
         qi = uniform(shfl(qi, 0));
         if (qi >= a.nq) break;
-        run_query<MODE, FILT, FAST, AUX>(a, L, V, qi, slot, M);
+        run_query<MODE, FILT, FAST, AUX, SH>(a, L, V, qi, slot, M);
     }
     // Leave the bitset clean for the next launch.
     if (V.dirty) visited_clear(a, V);
@@ -1149,22 +1268,23 @@ HM_DEV void search_loop(const SearchArgs &a) {
 // General kernel (any ef, filters): LDS-bound, so no occupancy target for the compiler.
 template <int MODE, bool FILT>
 __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
-    search_loop<MODE, FILT, false, 0>(a);
+    search_loop<MODE, FILT, false, 0, kShapeAny>(a);
 }
 
-// Register-beam kernel: four waves per SIMD (<= 128 VGPRs), as many as its LDS allows.
-template <int MODE, int AUX>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void search_fast_kernel(
+// Register-beam kernel: at least four waves per SIMD (<= 128 VGPRs), six (<= 80) for the
+// constant shape, whose LDS fits 24 waves per CU.
+template <int MODE, int AUX, int SH>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SH == kShapeM16D128 ? 6 : 4))) void search_fast_kernel(
     SearchArgs a) {
-    search_loop<MODE, false, true, AUX>(a);
+    search_loop<MODE, false, true, AUX, SH>(a);
 }
 
 // Register-beam kernel with the learned stop: the stop model's 18 values per lane do not fit
 // under 128 VGPRs beside the beam, so three waves per SIMD.
-template <int MODE>
+template <int MODE, int SH>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void search_stop_kernel(
     SearchArgs a) {
-    search_loop<MODE, false, true, 2>(a);
+    search_loop<MODE, false, true, 2, SH>(a);
 }
 
 __global__ void reset_queue_kernel(int32_t *q) { q[0] = 0; }
@@ -1174,13 +1294,15 @@ template __global__ void search_kernel<kExactU8, true>(SearchArgs);
 template __global__ void search_kernel<kSeqF32, false>(SearchArgs);
 template __global__ void search_kernel<kSeqF32, true>(SearchArgs);
 template __global__ void search_kernel<kSeqU8, false>(SearchArgs);
-template __global__ void search_fast_kernel<kExactU8, 0>(SearchArgs);
-template __global__ void search_fast_kernel<kSeqF32, 0>(SearchArgs);
-template __global__ void search_fast_kernel<kSeqU8, 0>(SearchArgs);
-template __global__ void search_fast_kernel<kExactU8, 1>(SearchArgs);
-template __global__ void search_fast_kernel<kSeqF32, 1>(SearchArgs);
-template __global__ void search_stop_kernel<kExactU8>(SearchArgs);
-template __global__ void search_stop_kernel<kSeqF32>(SearchArgs);
+template __global__ void search_fast_kernel<kExactU8, 0, kShapeAny>(SearchArgs);
+template __global__ void search_fast_kernel<kExactU8, 0, kShapeM16D128>(SearchArgs);
+template __global__ void search_fast_kernel<kSeqF32, 0, kShapeAny>(SearchArgs);
+template __global__ void search_fast_kernel<kSeqU8, 0, kShapeAny>(SearchArgs);
+template __global__ void search_fast_kernel<kExactU8, 1, kShapeAny>(SearchArgs);
+template __global__ void search_fast_kernel<kSeqF32, 1, kShapeAny>(SearchArgs);
+template __global__ void search_stop_kernel<kExactU8, kShapeAny>(SearchArgs);
+template __global__ void search_stop_kernel<kExactU8, kShapeM16D128>(SearchArgs);
+template __global__ void search_stop_kernel<kSeqF32, kShapeAny>(SearchArgs);
 
 }  // namespace hm
 
@@ -1190,24 +1312,35 @@ namespace hm {
 
 using KernelFn = void (*)(SearchArgs);
 
-static KernelFn pick(int mode, bool filt, bool fast, int aux) {
-    if (aux == 1) return mode == kExactU8 ? search_fast_kernel<kExactU8, 1> : search_fast_kernel<kSeqF32, 1>;
-    if (aux == 2) return mode == kExactU8 ? search_stop_kernel<kExactU8> : search_stop_kernel<kSeqF32>;
+static int aux_of(const SearchArgs &a) { return a.trace ? 1 : (a.stop_w ? 2 : 0); }
+
+static KernelFn pick(int mode, bool filt, const SearchArgs &a) {
+    const bool fast = a.fast != 0;
+    const int aux = aux_of(a);
+    const bool m16 = shape_m16d128(a, mode);
+    if (aux == 1)
+        return mode == kExactU8 ? search_fast_kernel<kExactU8, 1, kShapeAny>
+                                : search_fast_kernel<kSeqF32, 1, kShapeAny>;
+    if (aux == 2) {
+        if (mode != kExactU8) return search_stop_kernel<kSeqF32, kShapeAny>;
+        return m16 ? search_stop_kernel<kExactU8, kShapeM16D128>
+                   : search_stop_kernel<kExactU8, kShapeAny>;
+    }
     if (fast && !filt) {
-        if (mode == kExactU8) return search_fast_kernel<kExactU8, 0>;
-        if (mode == kSeqF32) return search_fast_kernel<kSeqF32, 0>;
-        return search_fast_kernel<kSeqU8, 0>;
+        if (mode == kExactU8)
+            return m16 ? search_fast_kernel<kExactU8, 0, kShapeM16D128>
+                       : search_fast_kernel<kExactU8, 0, kShapeAny>;
+        if (mode == kSeqF32) return search_fast_kernel<kSeqF32, 0, kShapeAny>;
+        return search_fast_kernel<kSeqU8, 0, kShapeAny>;
     }
     if (mode == kExactU8) return filt ? search_kernel<kExactU8, true> : search_kernel<kExactU8, false>;
     if (mode == kSeqF32) return filt ? search_kernel<kSeqF32, true> : search_kernel<kSeqF32, false>;
     return search_kernel<kSeqU8, false>;
 }
 
-static int aux_of(const SearchArgs &a) { return a.trace ? 1 : (a.stop_w ? 2 : 0); }
-
 // Blocks (one wave each) that fit on the device at once for this LDS size.
-int search_slots(int mode, bool filt, bool fast, int aux, int lds, int device_cus) {
-    KernelFn fn = pick(mode, filt, fast, aux);
+int search_slots(int mode, bool filt, const SearchArgs &a, int lds, int device_cus) {
+    KernelFn fn = pick(mode, filt, a);
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     int per_cu = 0;
@@ -1219,7 +1352,7 @@ int search_slots(int mode, bool filt, bool fast, int aux, int lds, int device_cu
 }
 
 hipError_t search_launch(int mode, bool filt, const SearchArgs &a, int blocks, hipStream_t s) {
-    KernelFn fn = pick(mode, filt, a.fast != 0, aux_of(a));
+    KernelFn fn = pick(mode, filt, a);
     if (a.lds_per_wave > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)fn,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -53,6 +53,8 @@ struct SearchArgs {
     uint32_t *bitsets;
     int64_t bitset_words;
     uint32_t *vlist;  // per wave slot: vlist_cap ids set in its bitset (clearing list)
+    int id_bits;  // node ids are < 2^id_bits
+    int probe_limit;  // > 0: cap on the probe distance of 16-bit visited slots
     int vlist_cap;
     uint64_t *spill_key;
     uint32_t *spill_id;
@@ -90,13 +92,23 @@ constexpr int kModelB2 = kModelW2 + kHidden;
 constexpr int kModelSize = kModelB2 + 1;
 constexpr int kTraceCols = kFeatures + 2;  // features, hits, |R|
 
+// Bytes of the visited hash with 1 << h_log2 slots: 16-bit quotient slots when the ids have at
+// most 10 bits more than the slot index, else 32-bit slots.
+inline int visited_hash_bytes(int h_log2, int id_bits) {
+    return (id_bits - h_log2 <= 10 ? 2 : 4) << h_log2;
+}
+
 // Bytes of LDS one wave of the register-beam kernel needs: query, 64-entry R scatter buffer
 // with its flags (shared with the new-neighbour staging), 64-entry tie list and the visited
-// hash. 128 dimensions with the 2,048-slot hash: 10,112 bytes, 16 waves per CU.
-inline int lds_bytes_fast(int mode, int dim, int dimp, int h_log2) {
+// hash (layer 0 with 1 << h_log2 slots, upper layers with at most 512). 128 dimensions, 1M ids
+// and 2,048 slots: 6,016 bytes (24 waves per CU fit).
+inline int lds_bytes_fast(int mode, int dim, int dimp, int h_log2, int id_bits) {
     auto up = [](int v) { return (v + 15) & ~15; };
     int q = mode == kExactU8 ? up(dimp) : up(dim * 8);
-    return q + 2 * (64 * 8 + 64 * 4) + 64 * 4 + (4 << h_log2);
+    const int upper = h_log2 < 9 ? h_log2 : 9;
+    const int h0 = visited_hash_bytes(h_log2, id_bits), hu = visited_hash_bytes(upper, id_bits);
+    const int hash = h0 > hu ? h0 : hu;
+    return q + 2 * (64 * 8 + 64 * 4) + 64 * 4 + hash;
 }
 
 // Bytes of LDS one wave needs for the given capacities.

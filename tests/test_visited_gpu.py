@@ -1,0 +1,113 @@
+"""GPU parity of the register-beam kernel's constant-shape build (M = 16, 128 one-byte
+dimensions) and of every form of its visited set: 16-bit quotient slots, 32-bit whole-id slots
+(ids with too many bits for the table), the move to the global bitset when the table fills or a
+probe runs out of distance bits (forced with a probe limit of 1), and other table sizes. Every
+variant must give the oracle's ids, distances and distance/hop counters bit for bit.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from hnsw_amd import _native, learned
+from hnsw_amd.engine import MI355XEngine
+from hnsw_amd.synthetic import clustered_u8
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(got, want, msg=""):
+    for g, w, name in zip(got, want, ("ids", "dists", "dcount", "hops")):
+        np.testing.assert_array_equal(g, w, err_msg=f"{msg} {name}")
+
+
+def _tune(eng, fast_hash_log2=0, probe_limit=0) -> None:
+    _native.check(_native.lib().hm_set_tuning(eng._device(), fast_hash_log2, 0, 0, probe_limit))
+
+
+def _moves(eng) -> int:
+    """Layer searches whose visited set moved to the global bitset since the last call."""
+    out = (ctypes.c_int32 * 2)()
+    _native.check(_native.lib().hm_read_stats(eng._device(), out, 1))
+    return int(out[0])
+
+
+def _build(n, dim, M, seed):
+    data, queries = clustered_u8(n, dim, 400, n_clusters=60, sigma=30.0, seed=seed)
+    levels = np.floor(-np.log(1.0 - np.random.default_rng(seed + 1).random(n))
+                      / np.log(M)).astype(np.int64)
+    eng = MI355XEngine(dim, "l2", M, 100, "heuristic", False, False)
+    eng.insert(data, levels)
+    return eng, data, queries
+
+
+@pytest.fixture(scope="module")
+def m16d128():
+    """20,000 x 128, M = 16: the constant-shape kernel; 15-bit ids, quotient slots."""
+    return _build(20_000, 128, 16, 31)
+
+
+@pytest.fixture(scope="module")
+def wide_ids():
+    """70,000 x 16, M = 8: 17-bit ids, so a 64-slot table needs 32-bit slots."""
+    return _build(70_000, 16, 8, 41)
+
+
+def _oracle(eng, queries, k, ef):
+    return oracle.search_batch(eng.native_arrays(), eng.entry_point, eng.max_level, "l2",
+                               queries, k, ef, threads=8)
+
+
+@pytest.mark.parametrize("ef,k", [(16, 10), (48, 10), (64, 64), (10, 1)])
+def test_constant_shape_kernel_matches_oracle(m16d128, ef: int, k: int) -> None:
+    eng, _, queries = m16d128
+    print(f"case m16d128 ef={ef} k={k}", flush=True)
+    _same(eng.search_many(queries, k, ef), _oracle(eng, queries, k, ef), f"ef={ef}")
+
+
+@pytest.mark.parametrize("fast_hash_log2,probe_limit", [(0, 1), (0, 3), (6, 0), (8, 0), (12, 0)])
+def test_visited_variants_match_oracle(m16d128, fast_hash_log2: int, probe_limit: int) -> None:
+    eng, _, queries = m16d128
+    want = _oracle(eng, queries, 10, 48)
+    print(f"case m16d128 hash_log2={fast_hash_log2} probe_limit={probe_limit}", flush=True)
+    _moves(eng)
+    _tune(eng, fast_hash_log2, probe_limit)
+    try:
+        got = eng.search_many(queries, 10, 48)
+        moves = _moves(eng)
+    finally:
+        _tune(eng)
+    _same(got, want, f"hash_log2={fast_hash_log2} probe_limit={probe_limit}")
+    if probe_limit == 1 or fast_hash_log2 == 6:
+        assert moves > 0  # the bitset path ran
+
+
+def test_whole_id_slots_match_oracle(wide_ids) -> None:
+    eng, _, queries = wide_ids
+    want = _oracle(eng, queries, 10, 32)
+    for hl in (6, 0):  # 6: 17 - 6 = 11 remainder bits, whole-id slots; default: quotient slots
+        print(f"case wide ids hash_log2={hl}", flush=True)
+        _tune(eng, hl)
+        try:
+            got = eng.search_many(queries, 10, 32)
+        finally:
+            _tune(eng)
+        _same(got, want, f"hash_log2={hl}")
+
+
+def test_constant_shape_stop_kernel_that_never_fires_is_the_plain_search(m16d128) -> None:
+    eng, _, queries = m16d128
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((2000, learned.N_FEATURES)).astype(np.float32)
+    p = learned.init_model(x, 3)
+    p[learned.MODEL_B2] = -1e4
+    eng.set_stop_model(p, 0.5)
+    try:
+        got = eng.search_many(queries, 10, 48, learned=True)
+    finally:
+        eng.set_stop_model(None)
+    _same(got, _oracle(eng, queries, 10, 48), "stop kernel")
