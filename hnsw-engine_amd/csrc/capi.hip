@@ -135,6 +135,7 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, 
         a.stop_w = h->stop_w.p;
         a.stop_logit = h->stop_logit;
         a.stop_tmin = h->stop_tmin;
+        a.lds_per_wave += kModelLdsBytes;
     }
     if (a.lds_per_wave > 160 * 1024) return invalid("ef too large for the on-chip search state");
     int slots = search_slots(mode, filt, a, a.lds_per_wave, h->cus);

@@ -40,6 +40,7 @@ struct Lds {
     int32_t *nbacc;
     uint32_t *nbal;
     uint32_t *hash;
+    uint32_t *model;  // learned stop: the packed model, kModelWords words per lane
     uint32_t *hist;  // 256 bins for the refill's radix select
     uint64_t *misc;  // [0..1]: first spilled entry
 };
@@ -77,7 +78,7 @@ HM_DEV Lds carve(uint8_t *base, const SearchArgs &a, int mode) {
 }
 
 // Compact layout of the register-beam kernel (fields it does not use stay null).
-HM_DEV Lds carve_fast(uint8_t *base, const SearchArgs &a, int mode) {
+HM_DEV Lds carve_fast(uint8_t *base, const SearchArgs &a, int mode, bool model) {
     auto up = [](int v) { return (v + 15) & ~15; };
     Lds L = {};
     uint8_t *p = base;
@@ -92,6 +93,10 @@ HM_DEV Lds carve_fast(uint8_t *base, const SearchArgs &a, int mode) {
     // stages R in rk[0] / ri[0] / nbal, so they share that space.
     L.nbi = L.ri[0];
     L.nbacc = (int32_t *)L.nbal;
+    if (model) {
+        L.model = (uint32_t *)p;
+        p += kModelLdsBytes;
+    }
     L.hash = (uint32_t *)p;
     return L;
 }
@@ -877,11 +882,11 @@ struct StopState {
 // The key of the candidate the search would expand next (the first unexpanded entry of R, else
 // the head of the tie list); the worst key of R when none is left.
 
-// The model as the search kernel holds it: the feature normalisation is folded into the first
-// layer at upload (w1' = w1 / std, b1' = b1 - sum w1' mean), so a lane keeps 18 values.
+// The model as the search kernel holds it: per lane (hidden unit) kModelWords words in the
+// wave's LDS, word i at w[i * 64 + lane]: words 0..7 the W1 column as bfloat16 pairs, word 8 b1
+// (low half) and W2 (high half).
 struct StopModel {
-    uint32_t w1[kFeatures / 2];  // this lane's hidden unit: W1 column as bfloat16 pairs
-    uint32_t b1w2;               // b1 (low half) and W2 (high half), bfloat16
+    const uint32_t *w;
     float b2;
 };
 
@@ -889,30 +894,43 @@ HM_DEV uint32_t bf16_bits(float x) {  // the value is already a bfloat16: keep i
     return __float_as_uint(x) >> 16;
 }
 
-HM_DEV void load_model(const SearchArgs &a, StopModel &M) {
+HM_DEV void load_model(const SearchArgs &a, const Lds &L, StopModel &M) {
     const int lane = lane_id();
 #pragma unroll
     for (int i = 0; i < kFeatures; i += 2)
-        M.w1[i / 2] = bf16_bits(a.stop_w[kModelW1 + i * kHidden + lane]) |
-                      (bf16_bits(a.stop_w[kModelW1 + (i + 1) * kHidden + lane]) << 16);
-    M.b1w2 = bf16_bits(a.stop_w[kModelB1 + lane]) | (bf16_bits(a.stop_w[kModelW2 + lane]) << 16);
+        L.model[(i / 2) * 64 + lane] =
+            bf16_bits(a.stop_w[kModelW1 + i * kHidden + lane]) |
+            (bf16_bits(a.stop_w[kModelW1 + (i + 1) * kHidden + lane]) << 16);
+    L.model[8 * 64 + lane] =
+        bf16_bits(a.stop_w[kModelB1 + lane]) | (bf16_bits(a.stop_w[kModelW2 + lane]) << 16);
+    M.w = L.model;
     M.b2 = a.stop_w[kModelB2];
+    wave_sync();
 }
 
 // 16 -> 64 (ReLU) -> 1 with float32 accumulation on normalised features (mean and inverse
 // standard deviation from the model, uniform across lanes); lane j owns hidden unit j.
 HM_DEV float model_logit(const SearchArgs &a, const StopModel &M, const float *f) {
-    float z = __uint_as_float(M.b1w2 << 16);
+    const int lane = lane_id();
+    uint32_t w[kModelWords];
+#pragma unroll
+    for (int i = 0; i < kModelWords; ++i) w[i] = M.w[i * 64 + lane];
+    float z = __uint_as_float(w[8] << 16);
 #pragma unroll
     for (int i = 0; i < kFeatures; i += 2) {
         const float x0 = (f[i] - a.stop_w[i]) * a.stop_w[kFeatures + i];
         const float x1 = (f[i + 1] - a.stop_w[i + 1]) * a.stop_w[kFeatures + i + 1];
-        z += __uint_as_float(M.w1[i / 2] << 16) * x0;
-        z += __uint_as_float(M.w1[i / 2] & 0xffff0000u) * x1;
+        z += __uint_as_float(w[i / 2] << 16) * x0;
+        z += __uint_as_float(w[i / 2] & 0xffff0000u) * x1;
     }
-    float y = __uint_as_float(M.b1w2 & 0xffff0000u) * fmaxf(z, 0.f);
-    for (int m = 32; m >= 1; m >>= 1) y += __shfl_xor(y, m, 64);
-    return y + M.b2;
+    const float y = __uint_as_float(w[8] & 0xffff0000u) * fmaxf(z, 0.f);
+    // Sum over the 64 hidden units: DPP sums within each row of 16 lanes, then the four row sums.
+    const float r = __int_as_float(group_sum_f32(y));
+    const float s01 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r), 0)) +
+                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r), 16));
+    const float s23 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r), 32)) +
+                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r), 48));
+    return (s01 + s23) + M.b2;
 }
 
 template <int MODE>
@@ -925,18 +943,20 @@ HM_DEV void features(const SearchArgs &a, const Query &Q, const StopState &S, ui
     const float dn = next_key == kInfKey ? dw : fdist<MODE>(next_key, a.metric);
     const float inv_span = __frcp_rn(fabsf(dk - dbest) + 1.f);
     const uint64_t topk = kk >= 64 ? ~0ull : ((1ull << kk) - 1ull);
+    // Every feature is wave-uniform: computed by the vector unit, kept in scalar registers.
+    auto u = [](float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); };
     f[0] = (float)S.t;
-    f[1] = __log2f(1.f + (float)S.dc0);
+    f[1] = u(__log2f(1.f + (float)S.dc0));
     f[2] = (float)Q.ef;
     f[3] = (float)Q.k;
-    f[4] = (float)rn * __frcp_rn((float)Q.ef);
-    f[5] = (float)popc(live) * __frcp_rn((float)Q.ef);
-    f[6] = slog(dbest);
-    f[7] = slog(dk);
-    f[8] = slog(dn);
-    f[9] = (dn - dk) * inv_span;
-    f[10] = (dk - dbest) * __frcp_rn(fabsf(dbest) + 1.f);
-    f[11] = (dw - dk) * inv_span;
+    f[4] = u((float)rn * __frcp_rn((float)Q.ef));
+    f[5] = u((float)popc(live) * __frcp_rn((float)Q.ef));
+    f[6] = u(slog(dbest));
+    f[7] = u(slog(dk));
+    f[8] = u(slog(dn));
+    f[9] = u((dn - dk) * inv_span);
+    f[10] = u((dk - dbest) * __frcp_rn(fabsf(dbest) + 1.f));
+    f[11] = u((dw - dk) * inv_span);
     f[12] = (float)S.since;
     f[13] = (float)S.last_acc;
     f[14] = (float)S.last_new;
@@ -1236,7 +1256,7 @@ template <int MODE, bool FILT, bool FAST, int AUX, int SH>
 HM_DEV void search_loop(const SearchArgs &a) {
     extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
     const int slot = blockIdx.x;
-    const Lds L = FAST ? carve_fast(lds_raw, a, MODE) : carve(lds_raw, a, MODE);
+    const Lds L = FAST ? carve_fast(lds_raw, a, MODE, AUX == 2) : carve(lds_raw, a, MODE);
     Visited V;
     V.bits = a.bitsets + (size_t)slot * a.bitset_words;
     V.list = a.vlist + (size_t)slot * a.vlist_cap;
@@ -1252,8 +1272,8 @@ HM_DEV void search_loop(const SearchArgs &a) {
     V.use_bits = false;
     V.hlog = a.h_log2;
     V.count = 0;
-    StopModel M;
-    if constexpr (AUX == 2) load_model(a, M);
+    StopModel M = {};
+    if constexpr (AUX == 2) load_model(a, L, M);
     for (;;) {
         int qi = 0;
         if (lane_id() == 0) qi = atomicAdd(a.queue, 1);
@@ -1279,10 +1299,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SH == kShape
     search_loop<MODE, false, true, AUX, SH>(a);
 }
 
-// Register-beam kernel with the learned stop: the stop model's 18 values per lane do not fit
-// under 128 VGPRs beside the beam, so three waves per SIMD.
+// Register-beam kernel with the learned stop (its model in LDS): five waves per SIMD
+// (<= 96 VGPRs) for the constant shape, four otherwise.
 template <int MODE, int SH>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void search_stop_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SH == kShapeM16D128 ? 5 : 4))) void search_stop_kernel(
     SearchArgs a) {
     search_loop<MODE, false, true, 2, SH>(a);
 }

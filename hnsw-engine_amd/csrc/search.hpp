@@ -90,6 +90,8 @@ constexpr int kModelB1 = kModelW1 + kFeatures * kHidden;
 constexpr int kModelW2 = kModelB1 + kHidden;
 constexpr int kModelB2 = kModelW2 + kHidden;
 constexpr int kModelSize = kModelB2 + 1;
+constexpr int kModelWords = kFeatures / 2 + 1;        // packed bfloat16 words per hidden unit
+constexpr int kModelLdsBytes = kModelWords * 64 * 4;  // the search kernel's LDS copy
 constexpr int kTraceCols = kFeatures + 2;  // features, hits, |R|
 
 // Bytes of the visited hash with 1 << h_log2 slots: 16-bit quotient slots when the ids have at

@@ -69,6 +69,21 @@ HM_DEV int group_sum(int v, int g) {
     return v;
 }
 
+// Float sum over each row of 16 lanes, every lane of the row receiving it (as bits): the same
+// DPP exchanges as group_sum with g = 16.
+HM_DEV int group_sum_f32(float x) {
+    int v = __float_as_int(x);
+    v = __float_as_int(__int_as_float(v) +
+                       __int_as_float(__builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false)));
+    v = __float_as_int(__int_as_float(v) +
+                       __int_as_float(__builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false)));
+    v = __float_as_int(__int_as_float(v) +
+                       __int_as_float(__builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false)));
+    v = __float_as_int(__int_as_float(v) +
+                       __int_as_float(__builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false)));
+    return v;
+}
+
 HM_DEV int wave_sum(int v) {
     for (int m = 32; m >= 1; m >>= 1) v += shfl_xor(v, m);
     return v;
