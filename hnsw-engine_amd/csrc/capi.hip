@@ -82,7 +82,8 @@ void plan(SearchArgs &a, int mode, const hm_index *h, int max_ef, bool fast) {
     a.c_cap = 1 << std::min(12, std::max(6, ceil_log2(2 * (int64_t)max_ef)));
     a.h_log2 = h->tune_hlog ? h->tune_hlog
                             : std::min(12, std::max(10, ceil_log2((int64_t)max_ef * 64)));
-    a.lds_per_wave = lds_bytes_per_wave(mode, h->dim, h->dimp, a.r_cap, a.c_cap, a.h_log2);
+    a.lds_per_wave =
+        lds_bytes_per_wave(mode, h->dim, h->dimp, a.r_cap, a.c_cap, a.h_log2, a.id_bits);
 }
 
 int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, int max_slots,

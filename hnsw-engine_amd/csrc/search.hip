@@ -71,7 +71,13 @@ HM_DEV Lds carve(uint8_t *base, const SearchArgs &a, int mode) {
     L.nbi = (uint32_t *)p; p += 64 * 4;
     L.nbacc = (int32_t *)p; p += 64 * 4;
     L.nbal = (uint32_t *)p; p += 64 * 4;
-    L.hash = (uint32_t *)p; p += 4 << a.h_log2;
+    L.hash = (uint32_t *)p;
+    {
+        const int upper = a.h_log2 < 9 ? a.h_log2 : 9;
+        const int h0 = visited_hash_bytes(a.h_log2, a.id_bits);
+        const int hu = visited_hash_bytes(upper, a.id_bits);
+        p += h0 > hu ? h0 : hu;
+    }
     L.hist = (uint32_t *)p; p += 256 * 4;
     L.misc = (uint64_t *)p;
     return L;
@@ -299,8 +305,8 @@ HM_DEV double decode(uint64_t k, int metric) {
 
 // Visited set of one layer search: an open-addressing hash in LDS (linear probing) until it is
 // half full, then a per-wave global bitset. Whole-id mode: one 32-bit slot per id (id + 1).
-// Quotient mode (register-beam kernel, when the ids have few bits more than the table index):
-// 16-bit slots, two per word. The id is first mapped by a bijection of [0, 2^B) (multiplication
+// Quotient mode (when the ids have at most 10 bits more than the table index): 16-bit slots,
+// two per word. The id is first mapped by a bijection of [0, 2^B) (multiplication
 // by an odd constant modulo 2^B, B = max(id bits, hlog)); the top hlog bits of the image are the
 // home slot, the low r = B - hlog bits (the remainder) are stored with the probe distance from
 // home: slot = (remainder << (16 - r)) | (distance + 1). A slot therefore determines its id
@@ -317,8 +323,7 @@ struct Visited {
     int rbits;    // quotient mode: remainder bits
     int probe_limit;  // > 0: quotient mode moves to the bitset beyond this probe distance
     bool use_bits, dirty;
-    bool quotient_ok;  // this kernel's LDS holds 16-bit slots
-    bool quotient;     // the current layer search uses them
+    bool quotient;  // the current layer search uses 16-bit slots
 };
 
 constexpr uint32_t kHashMul = 2654435761u;
@@ -366,7 +371,7 @@ HM_DEV void visited_reset(const SearchArgs &a, const Lds &L, Visited &V, int hlo
     V.hlog = hlog;
     V.count = 0;
     V.rbits = V.id_bits > hlog ? V.id_bits - hlog : 0;
-    V.quotient = V.quotient_ok && V.rbits <= kQuotientMaxRemainder;
+    V.quotient = V.rbits <= kQuotientMaxRemainder;
     const int words = V.quotient ? (1 << hlog) / 2 : (1 << hlog);
     for (int i = lane; i < words; i += 64) L.hash[i] = 0u;
     wave_sync();
@@ -1260,7 +1265,6 @@ HM_DEV void search_loop(const SearchArgs &a) {
     Visited V;
     V.bits = a.bitsets + (size_t)slot * a.bitset_words;
     V.list = a.vlist + (size_t)slot * a.vlist_cap;
-    V.quotient_ok = FAST;
     V.quotient = false;
     V.id_bits = a.id_bits;
     V.rbits = 0;

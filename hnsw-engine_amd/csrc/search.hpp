@@ -96,7 +96,7 @@ constexpr int kTraceCols = kFeatures + 2;  // features, hits, |R|
 
 // Bytes of the visited hash with 1 << h_log2 slots: 16-bit quotient slots when the ids have at
 // most 10 bits more than the slot index, else 32-bit slots.
-inline int visited_hash_bytes(int h_log2, int id_bits) {
+__host__ __device__ inline int visited_hash_bytes(int h_log2, int id_bits) {
     return (id_bits - h_log2 <= 10 ? 2 : 4) << h_log2;
 }
 
@@ -114,13 +114,16 @@ inline int lds_bytes_fast(int mode, int dim, int dimp, int h_log2, int id_bits) 
 }
 
 // Bytes of LDS one wave needs for the given capacities.
-inline int lds_bytes_per_wave(int mode, int dim, int dimp, int r_cap, int c_cap, int h_log2) {
+inline int lds_bytes_per_wave(int mode, int dim, int dimp, int r_cap, int c_cap, int h_log2,
+                              int id_bits) {
     auto up = [](int v) { return (v + 15) & ~15; };
     int q = mode == kExactU8 ? up(dimp) : up(dim * 8);
     int r = 2 * (up(r_cap * 8) + up(r_cap * 4));
     int c = 2 * (up(c_cap * 8) + up(c_cap * 4));
     int stage = 2 * (64 * 8 + 64 * 4) + 64 * 8 + 64 * 4 + 64 * 4 + 64 * 4;
-    int hash = 4 << h_log2;
+    const int upper = h_log2 < 9 ? h_log2 : 9;
+    const int h0 = visited_hash_bytes(h_log2, id_bits), hu = visited_hash_bytes(upper, id_bits);
+    int hash = h0 > hu ? h0 : hu;
     int misc = 256 * 4 + 64;
     return q + r + c + stage + hash + misc;
 }
