@@ -172,14 +172,18 @@ HM_DEV void exact_rows(const SearchArgs &a, const Lds &L, int nnew, int lg_rt) {
     const int vpi = 64 >> lg;
     constexpr int kBatch = 4;
     for (int v0 = 0; v0 < nnew; v0 += vpi * kBatch) {
+        // All ids first, then the row loads: every load is in flight before the first is used.
+        // Lanes past the last row load that row again (their sums are not stored), so the loads
+        // take no per-lane branch.
+        uint32_t ids[kBatch];
+#pragma unroll
+        for (int it = 0; it < kBatch; ++it)
+            ids[it] = L.nbi[min(v0 + it * vpi + (lane >> lg), nnew - 1)];
         uint4 xs[kBatch];
 #pragma unroll
         for (int it = 0; it < kBatch; ++it) {
-            int v = v0 + it * vpi + (lane >> lg);
-            if (v0 + it * vpi < nnew && v < nnew) {
-                uint32_t id = L.nbi[v];
-                xs[it] = *(const uint4 *)(a.vec_u8 + ((size_t)id << (lg + 4)) + (size_t)c * 16);
-            }
+            if (v0 + it * vpi < nnew)
+                xs[it] = *(const uint4 *)(a.vec_u8 + ((size_t)ids[it] << (lg + 4)) + (size_t)c * 16);
         }
 #pragma unroll
         for (int it = 0; it < kBatch; ++it) {
