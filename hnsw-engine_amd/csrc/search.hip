@@ -1178,6 +1178,204 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
     return true;
 }
 
+// layer0_fast for the constant shape (exact 8-bit distances, whose keys fit in 32 bits): an
+// entry of R is one 64-bit word, key << 32 | id, so that (key, id) order is the word's order
+// and a key comparison reads the high half. Same algorithm, fewer registers and compares.
+HM_DEV uint32_t key_hi(uint64_t e) { return (uint32_t)(e >> 32); }
+
+template <int AUX>
+HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
+                          uint64_t entry_k, uint32_t entry_id, uint64_t &rk_out,
+                          uint32_t &ri_out, int &rn, int qi, const StopModel &M) {
+    constexpr int MODE = kExactU8;
+    constexpr uint64_t kInfEntry = ~0ull;
+    const int lane = lane_id();
+    StopState S;
+    const int ef = Q.ef;
+    constexpr int kTies = 64;
+    uint64_t *const tk = L.ck[0];  // evicted ties (key, id), sorted, [th, tn)
+    uint32_t *const ti = L.ci[0];
+    visited_reset(a, L, V, a.h_log2);
+    visit_lanes(L, V, entry_id, lane == 0);
+    V.count = 1;
+    wave_sync();
+    uint64_t re = lane == 0 ? (entry_k << 32) | entry_id : kInfEntry;
+    rn = 1;
+    uint64_t expd = 0;
+    int th = 0, tn = 0;
+    for (;;) {
+        const uint64_t valid = rn >= 64 ? ~0ull : ((1ull << rn) - 1ull);
+        const uint64_t live = valid & ~expd;
+        uint32_t cid;
+        if (live) {
+            const int p = __ffsll((long long)live) - 1;
+            cid = readlane_u32((uint32_t)re, p);
+            expd |= 1ull << p;
+        } else if (th < tn) {
+            cid = ti[th];
+            th += 1;
+        } else {
+            break;
+        }
+        Q.hops += 1;
+        if constexpr (AUX != 0) {
+            S.last_new = 0;
+            S.last_acc = 0;
+        }
+        const int32_t nb = lane < 32 ? a.links0[(size_t)cid * 32 + lane] : -1;
+        visited_make_room(L, V, 32);
+        const bool isnew = visit_lanes(L, V, (uint32_t)nb, nb >= 0);
+        visited_note(V, isnew, (uint32_t)nb);
+        const uint64_t nm = ballot(isnew);
+        const int nnew = popc(nm);
+        if (nnew == 0) goto expanded;
+        {
+            if (isnew) L.nbi[rank_below(nm)] = (uint32_t)nb;
+            wave_sync();
+            V.count += nnew;
+            const uint64_t ki = distances<MODE, kShapeM16D128>(a, L, nnew, Q.qq);
+            const uint64_t ne = lane < nnew ? (ki << 32) | L.nbi[lane] : kInfEntry;
+            const uint32_t nk = key_hi(ne);
+            Q.dcount += nnew;
+            if constexpr (AUX != 0) {
+                S.dc0 += nnew;
+                S.last_new += nnew;
+            }
+            // Acceptance as in layer0_fast: only neighbours below the worst key of a full R.
+            const uint32_t rkey = key_hi(re);
+            const uint64_t all_new = (1ull << nnew) - 1ull;  // nnew <= 32
+            const uint32_t worst = readlane_u32(rkey, ef - 1);
+            const uint64_t part = rn == ef ? ballot(lane < nnew && nk < worst) : all_new;
+            int c_r = 0, c_n = 0;
+            for (uint64_t m = part; m; m &= m - 1ull) {
+                const int q = __ffsll((long long)m) - 1;
+                const uint32_t kq = readlane_u32(nk, q);
+                const int cnt = popc(ballot(lane < rn && rkey <= kq));
+                if (lane == q) c_r = cnt;
+                if (q < lane && kq <= nk) c_n += 1;
+            }
+            const bool acc = ((part >> lane) & 1ull) && c_r + c_n < ef;
+            const uint64_t am = ballot(acc);
+            if (am == 0) goto expanded;
+            const int na = popc(am);
+            if constexpr (AUX != 0) S.last_acc += na;
+            int pos_r = lane, r_below = 0, a_below = 0;
+            for (uint64_t m = am; m; m &= m - 1ull) {
+                const int q = __ffsll((long long)m) - 1;
+                const uint64_t eq = readlane_u64(ne, q);
+                if (eq < re) pos_r += 1;
+                const int below = popc(ballot(lane < rn && re < eq));
+                if (lane == q) r_below = below;
+                if (acc && eq < ne) a_below += 1;
+            }
+            const int pos_a = r_below + a_below;
+            const int total = rn + na;
+            const int nrn = min(ef, total);
+            const bool rflag = (expd >> lane) & 1ull;
+            if (lane < rn && pos_r < ef) {
+                L.rk[0][pos_r] = re;
+                L.nbal[pos_r] = rflag ? 1u : 0u;
+            }
+            if (acc && pos_a < ef) {
+                L.rk[0][pos_a] = ne;
+                L.nbal[pos_a] = 0u;
+            }
+            wave_sync();
+            const uint64_t nre = lane < nrn ? L.rk[0][lane] : kInfEntry;
+            const bool nflag = lane < nrn && L.nbal[lane] != 0u;
+            if (total > ef) {
+                // Evictions: keep the unexpanded ones whose key equals the new worst.
+                const uint32_t nwk = key_hi(readlane_u64(nre, ef - 1));
+                const bool was_full = rn == ef;
+                if (!was_full || nwk < worst) {
+                    th = 0;
+                    tn = 0;
+                }
+                const bool er = lane < rn && pos_r >= ef && !rflag && rkey == nwk;
+                const bool ea = acc && pos_a >= ef && nk == nwk;
+                const uint64_t em = ballot(er || ea);
+                if (em) {
+                    const int nev = popc(em);
+                    const int live_t = tn - th;
+                    if (live_t + nev > kTies) {
+                        if (lane == 0 && a.stats) atomicAdd(&a.stats[1], 1);
+                        return false;
+                    }
+                    uint64_t k0 = 0;
+                    uint32_t i0 = 0;
+                    if (lane < live_t) {
+                        k0 = tk[th + lane];
+                        i0 = ti[th + lane];
+                    }
+                    wave_sync();
+                    if (lane < live_t) {
+                        tk[lane] = k0;
+                        ti[lane] = i0;
+                    }
+                    wave_sync();
+                    if (er || ea) {
+                        const int d = live_t + rank_below(em);
+                        const uint64_t e = er ? re : ne;
+                        tk[d] = key_hi(e);
+                        ti[d] = (uint32_t)e;
+                    }
+                    th = 0;
+                    tn = live_t + nev;
+                    int p2 = 1;
+                    while (p2 < tn) p2 <<= 1;
+                    bitonic_sort(tk, ti, tn, p2);
+                }
+            }
+            wave_sync();
+            re = nre;
+            expd = ballot(nflag);
+            rn = nrn;
+        }
+    expanded:
+        if constexpr (AUX != 0) {
+            S.t += 1;
+            const uint64_t rk = lane < rn ? (uint64_t)key_hi(re) : kInfKey;
+            const int kk = max(1, min(Q.k, rn));
+            const uint64_t kth = readlane_u64(rk, kk - 1);
+            S.since = kth == S.last_kth ? S.since + 1 : 0;
+            S.last_kth = kth;
+            if (AUX == 2 && (S.t < a.stop_tmin || (S.t & 1) != 0)) continue;
+            const uint64_t valid_now = rn >= 64 ? ~0ull : ((1ull << rn) - 1ull);
+            const uint64_t live_now = valid_now & ~expd;
+            uint64_t next_key = kInfKey;
+            if (live_now) next_key = readlane_u64(rk, __ffsll((long long)live_now) - 1);
+            else if (th < tn) next_key = tk[th];
+            float f[kFeatures];
+            features<MODE>(a, Q, S, rk, rn, live_now, next_key, f);
+            if constexpr (AUX == 1) {
+                const double thr = a.trace_thr[qi];
+                const int hits = popc(ballot(lane < kk && decode<MODE>(rk, a.metric) <= thr));
+                if (S.t <= a.trace_steps) {
+                    float *row = a.trace + ((size_t)qi * a.trace_steps + (S.t - 1)) * kTraceCols;
+                    if (lane < kFeatures) {
+                        float v = 0.f;
+#pragma unroll
+                        for (int i = 0; i < kFeatures; ++i)
+                            if (lane == i) v = f[i];
+                        row[lane] = v;
+                    }
+                    if (lane == kFeatures) row[lane] = (float)hits;
+                    if (lane == kFeatures + 1) row[lane] = (float)rn;
+                }
+                if (lane == 0) {
+                    a.trace_len[qi] = S.t;
+                    a.trace_final[qi] = hits;
+                }
+            } else if (model_logit(a, M, f) > a.stop_logit) {
+                break;
+            }
+        }
+    }
+    rk_out = lane < rn ? (uint64_t)key_hi(re) : kInfKey;
+    ri_out = lane < rn ? (uint32_t)re : kInfId;
+    return true;
+}
+
 // ---------------------------------------------------------------------------- one query
 
 template <int MODE, bool FILT, bool FAST, int AUX, int SH>
@@ -1230,7 +1428,10 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
     if constexpr (FAST) {
         uint64_t rk;
         uint32_t ri;
-        ok = layer0_fast<MODE, AUX, SH>(a, L, V, Q, cur_k, cur_id, rk, ri, rn, qi, M);
+        if constexpr (SH == kShapeM16D128)
+            ok = layer0_packed<AUX>(a, L, V, Q, cur_k, cur_id, rk, ri, rn, qi, M);
+        else
+            ok = layer0_fast<MODE, AUX, SH>(a, L, V, Q, cur_k, cur_id, rk, ri, rn, qi, M);
         if (!ok) rn = 0;
         const int take = min(Q.k, rn);
         for (int i = lane; i < a.kstride; i += 64) {
