@@ -90,11 +90,11 @@ HM_DEV Lds carve_fast(uint8_t *base, const SearchArgs &a, int mode, bool model) 
     uint8_t *p = base;
     L.qb = p;
     p += mode == kExactU8 ? up(a.dimp) : up(a.dim * 8);
-    L.rk[0] = (uint64_t *)p; p += 64 * 8;
+    L.rk[0] = (uint64_t *)p; p += 64 * 8 * a.nr;
     L.ck[0] = (uint64_t *)p; p += 64 * 8;
     L.ri[0] = (uint32_t *)p; p += 64 * 4;
     L.ci[0] = (uint32_t *)p; p += 64 * 4;
-    L.nbal = (uint32_t *)p; p += 64 * 4;
+    L.nbal = (uint32_t *)p; p += 64 * 4 * a.nr;
     // One expansion reads the new ids (nbi) and the partial sums (nbacc) before its merge
     // stages R in rk[0] / ri[0] / nbal, so they share that space.
     L.nbi = L.ri[0];
@@ -112,9 +112,11 @@ HM_DEV Lds carve_fast(uint8_t *base, const SearchArgs &a, int mode, bool model) 
 // one-byte dimensions), where every loop bound and stride is a constant.
 constexpr int kShapeAny = 0;
 constexpr int kShapeM16D128 = 1;
+constexpr int kShapeM16D128Wide = 2;  // the same shape with two R entries per lane (ef <= 128)
 
-template <int SH> HM_DEV int w0_of(const SearchArgs &a) { return SH == kShapeM16D128 ? 32 : a.w0; }
-template <int SH> HM_DEV int w1_of(const SearchArgs &a) { return SH == kShapeM16D128 ? 16 : a.w1; }
+template <int SH> constexpr int nr_of() { return SH == kShapeM16D128Wide ? 2 : 1; }
+template <int SH> HM_DEV int w0_of(const SearchArgs &a) { return SH != kShapeAny ? 32 : a.w0; }
+template <int SH> HM_DEV int w1_of(const SearchArgs &a) { return SH != kShapeAny ? 16 : a.w1; }
 
 inline bool shape_m16d128(const SearchArgs &a, int mode) {
     return mode == kExactU8 && a.w0 == 32 && a.w1 == 16 && a.dimp == 128;
@@ -204,7 +206,7 @@ HM_DEV uint64_t dist_exact(const SearchArgs &a, const Lds &L, int nnew, int32_t 
     const int lane = lane_id();
     const int cpv = a.dimp >> 4;
     const uint4 *qv = (const uint4 *)L.qb;
-    if constexpr (SH == kShapeM16D128) {
+    if constexpr (SH != kShapeAny) {
         exact_rows<3>(a, L, nnew, 3);
     } else if ((cpv & (cpv - 1)) == 0 && cpv <= 64) {
         exact_rows<-1>(a, L, nnew, __builtin_ctz(cpv));
@@ -1178,15 +1180,16 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
     return true;
 }
 
-// layer0_fast for the constant shape (exact 8-bit distances, whose keys fit in 32 bits): an
+// layer0_fast for the constant shapes (exact 8-bit distances, whose keys fit in 32 bits): an
 // entry of R is one 64-bit word, key << 32 | id, so that (key, id) order is the word's order
-// and a key comparison reads the high half. Same algorithm, fewer registers and compares.
+// and a key comparison reads the high half. Same algorithm, fewer registers and compares. R
+// entry e lives in lane e % 64 of slot e / 64: NR = 1 holds ef <= 64, NR = 2 ef <= 128.
 HM_DEV uint32_t key_hi(uint64_t e) { return (uint32_t)(e >> 32); }
 
-template <int AUX>
+template <int AUX, int NR>
 HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
-                          uint64_t entry_k, uint32_t entry_id, uint64_t &rk_out,
-                          uint32_t &ri_out, int &rn, int qi, const StopModel &M) {
+                          uint64_t entry_k, uint32_t entry_id, uint64_t *rk_out,
+                          uint32_t *ri_out, int &rn, int qi, const StopModel &M) {
     constexpr int MODE = kExactU8;
     constexpr uint64_t kInfEntry = ~0ull;
     const int lane = lane_id();
@@ -1199,23 +1202,36 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
     visit_lanes(L, V, entry_id, lane == 0);
     V.count = 1;
     wave_sync();
-    uint64_t re = lane == 0 ? (entry_k << 32) | entry_id : kInfEntry;
+    uint64_t re[NR], expd[NR];
+#pragma unroll
+    for (int s = 0; s < NR; ++s) {
+        re[s] = kInfEntry;
+        expd[s] = 0;
+    }
+    if (lane == 0) re[0] = (entry_k << 32) | entry_id;
     rn = 1;
-    uint64_t expd = 0;
     int th = 0, tn = 0;
+    // entry lane + 64 s of R exists
+    auto in_r = [&](int s, int n) { return lane + 64 * s < n; };
     for (;;) {
-        const uint64_t valid = rn >= 64 ? ~0ull : ((1ull << rn) - 1ull);
-        const uint64_t live = valid & ~expd;
-        uint32_t cid;
-        if (live) {
-            const int p = __ffsll((long long)live) - 1;
-            cid = readlane_u32((uint32_t)re, p);
-            expd |= 1ull << p;
-        } else if (th < tn) {
+        uint32_t cid = 0;
+        bool have = false;
+#pragma unroll
+        for (int s = 0; s < NR; ++s) {
+            const int cnt = min(max(rn - 64 * s, 0), 64);
+            const uint64_t valid = cnt >= 64 ? ~0ull : ((1ull << cnt) - 1ull);
+            const uint64_t live = valid & ~expd[s];
+            if (!have && live) {
+                const int p = __ffsll((long long)live) - 1;
+                cid = readlane_u32((uint32_t)re[s], p);
+                expd[s] |= 1ull << p;
+                have = true;
+            }
+        }
+        if (!have) {
+            if (th >= tn) break;
             cid = ti[th];
             th += 1;
-        } else {
-            break;
         }
         Q.hops += 1;
         if constexpr (AUX != 0) {
@@ -1242,15 +1258,19 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
                 S.last_new += nnew;
             }
             // Acceptance as in layer0_fast: only neighbours below the worst key of a full R.
-            const uint32_t rkey = key_hi(re);
+            uint32_t rkey[NR];
+#pragma unroll
+            for (int s = 0; s < NR; ++s) rkey[s] = key_hi(re[s]);
             const uint64_t all_new = (1ull << nnew) - 1ull;  // nnew <= 32
-            const uint32_t worst = readlane_u32(rkey, ef - 1);
+            const uint32_t worst = readlane_u32(rkey[(ef - 1) >> 6], (ef - 1) & 63);
             const uint64_t part = rn == ef ? ballot(lane < nnew && nk < worst) : all_new;
             int c_r = 0, c_n = 0;
             for (uint64_t m = part; m; m &= m - 1ull) {
                 const int q = __ffsll((long long)m) - 1;
                 const uint32_t kq = readlane_u32(nk, q);
-                const int cnt = popc(ballot(lane < rn && rkey <= kq));
+                int cnt = 0;
+#pragma unroll
+                for (int s = 0; s < NR; ++s) cnt += popc(ballot(in_r(s, rn) && rkey[s] <= kq));
                 if (lane == q) c_r = cnt;
                 if (q < lane && kq <= nk) c_n += 1;
             }
@@ -1259,43 +1279,64 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
             if (am == 0) goto expanded;
             const int na = popc(am);
             if constexpr (AUX != 0) S.last_acc += na;
-            int pos_r = lane, r_below = 0, a_below = 0;
+            int pos_r[NR];
+#pragma unroll
+            for (int s = 0; s < NR; ++s) pos_r[s] = lane + 64 * s;
+            int r_below = 0, a_below = 0;
             for (uint64_t m = am; m; m &= m - 1ull) {
                 const int q = __ffsll((long long)m) - 1;
                 const uint64_t eq = readlane_u64(ne, q);
-                if (eq < re) pos_r += 1;
-                const int below = popc(ballot(lane < rn && re < eq));
+                int below = 0;
+#pragma unroll
+                for (int s = 0; s < NR; ++s) {
+                    if (eq < re[s]) pos_r[s] += 1;
+                    below += popc(ballot(in_r(s, rn) && re[s] < eq));
+                }
                 if (lane == q) r_below = below;
                 if (acc && eq < ne) a_below += 1;
             }
             const int pos_a = r_below + a_below;
             const int total = rn + na;
             const int nrn = min(ef, total);
-            const bool rflag = (expd >> lane) & 1ull;
-            if (lane < rn && pos_r < ef) {
-                L.rk[0][pos_r] = re;
-                L.nbal[pos_r] = rflag ? 1u : 0u;
+            bool rflag[NR];
+#pragma unroll
+            for (int s = 0; s < NR; ++s) {
+                rflag[s] = (expd[s] >> lane) & 1ull;
+                if (in_r(s, rn) && pos_r[s] < ef) {
+                    L.rk[0][pos_r[s]] = re[s];
+                    L.nbal[pos_r[s]] = rflag[s] ? 1u : 0u;
+                }
             }
             if (acc && pos_a < ef) {
                 L.rk[0][pos_a] = ne;
                 L.nbal[pos_a] = 0u;
             }
             wave_sync();
-            const uint64_t nre = lane < nrn ? L.rk[0][lane] : kInfEntry;
-            const bool nflag = lane < nrn && L.nbal[lane] != 0u;
+            uint64_t nre[NR];
+            bool nflag[NR];
+#pragma unroll
+            for (int s = 0; s < NR; ++s) {
+                nre[s] = in_r(s, nrn) ? L.rk[0][lane + 64 * s] : kInfEntry;
+                nflag[s] = in_r(s, nrn) && L.nbal[lane + 64 * s] != 0u;
+            }
             if (total > ef) {
-                // Evictions: keep the unexpanded ones whose key equals the new worst.
-                const uint32_t nwk = key_hi(readlane_u64(nre, ef - 1));
+                // Evictions: keep the unexpanded ones whose key equals the new worst (an R entry
+                // and a new one of the same lane can both be such).
+                const uint32_t nwk = key_hi(readlane_u64(nre[(ef - 1) >> 6], (ef - 1) & 63));
                 const bool was_full = rn == ef;
                 if (!was_full || nwk < worst) {
                     th = 0;
                     tn = 0;
                 }
-                const bool er = lane < rn && pos_r >= ef && !rflag && rkey == nwk;
-                const bool ea = acc && pos_a >= ef && nk == nwk;
-                const uint64_t em = ballot(er || ea);
-                if (em) {
-                    const int nev = popc(em);
+                uint64_t em[NR + 1];
+#pragma unroll
+                for (int s = 0; s < NR; ++s)
+                    em[s] = ballot(in_r(s, rn) && pos_r[s] >= ef && !rflag[s] && rkey[s] == nwk);
+                em[NR] = ballot(acc && pos_a >= ef && nk == nwk);
+                int nev = 0;
+#pragma unroll
+                for (int s = 0; s <= NR; ++s) nev += popc(em[s]);
+                if (nev) {
                     const int live_t = tn - th;
                     if (live_t + nev > kTies) {
                         if (lane == 0 && a.stats) atomicAdd(&a.stats[1], 1);
@@ -1313,11 +1354,15 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
                         ti[lane] = i0;
                     }
                     wave_sync();
-                    if (er || ea) {
-                        const int d = live_t + rank_below(em);
-                        const uint64_t e = er ? re : ne;
-                        tk[d] = key_hi(e);
-                        ti[d] = (uint32_t)e;
+                    int d0 = live_t;
+#pragma unroll
+                    for (int s = 0; s <= NR; ++s) {
+                        if ((em[s] >> lane) & 1ull) {
+                            const uint64_t e = s < NR ? re[s < NR ? s : 0] : ne;
+                            tk[d0 + rank_below(em[s])] = key_hi(e);
+                            ti[d0 + rank_below(em[s])] = (uint32_t)e;
+                        }
+                        d0 += popc(em[s]);
                     }
                     th = 0;
                     tn = live_t + nev;
@@ -1327,21 +1372,25 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
                 }
             }
             wave_sync();
-            re = nre;
-            expd = ballot(nflag);
+#pragma unroll
+            for (int s = 0; s < NR; ++s) {
+                re[s] = nre[s];
+                expd[s] = ballot(nflag[s]);
+            }
             rn = nrn;
         }
     expanded:
         if constexpr (AUX != 0) {
+            static_assert(NR == 1, "the stop model and traces read a one-slot beam");
             S.t += 1;
-            const uint64_t rk = lane < rn ? (uint64_t)key_hi(re) : kInfKey;
+            const uint64_t rk = lane < rn ? (uint64_t)key_hi(re[0]) : kInfKey;
             const int kk = max(1, min(Q.k, rn));
             const uint64_t kth = readlane_u64(rk, kk - 1);
             S.since = kth == S.last_kth ? S.since + 1 : 0;
             S.last_kth = kth;
             if (AUX == 2 && (S.t < a.stop_tmin || (S.t & 1) != 0)) continue;
             const uint64_t valid_now = rn >= 64 ? ~0ull : ((1ull << rn) - 1ull);
-            const uint64_t live_now = valid_now & ~expd;
+            const uint64_t live_now = valid_now & ~expd[0];
             uint64_t next_key = kInfKey;
             if (live_now) next_key = readlane_u64(rk, __ffsll((long long)live_now) - 1);
             else if (th < tn) next_key = tk[th];
@@ -1371,8 +1420,11 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
             }
         }
     }
-    rk_out = lane < rn ? (uint64_t)key_hi(re) : kInfKey;
-    ri_out = lane < rn ? (uint32_t)re : kInfId;
+#pragma unroll
+    for (int s = 0; s < NR; ++s) {
+        rk_out[s] = in_r(s, rn) ? (uint64_t)key_hi(re[s]) : kInfKey;
+        ri_out[s] = in_r(s, rn) ? (uint32_t)re[s] : kInfId;
+    }
     return true;
 }
 
@@ -1426,18 +1478,23 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
     int rn = 0;
     bool ok;
     if constexpr (FAST) {
-        uint64_t rk;
-        uint32_t ri;
-        if constexpr (SH == kShapeM16D128)
-            ok = layer0_packed<AUX>(a, L, V, Q, cur_k, cur_id, rk, ri, rn, qi, M);
+        constexpr int NR = nr_of<SH>();
+        uint64_t rk[NR];
+        uint32_t ri[NR];
+        if constexpr (SH != kShapeAny)
+            ok = layer0_packed<AUX, NR>(a, L, V, Q, cur_k, cur_id, rk, ri, rn, qi, M);
         else
-            ok = layer0_fast<MODE, AUX, SH>(a, L, V, Q, cur_k, cur_id, rk, ri, rn, qi, M);
+            ok = layer0_fast<MODE, AUX, SH>(a, L, V, Q, cur_k, cur_id, rk[0], ri[0], rn, qi, M);
         if (!ok) rn = 0;
         const int take = min(Q.k, rn);
-        for (int i = lane; i < a.kstride; i += 64) {
-            const bool own = i < take;  // lane i holds R[i] (kstride <= 64 on this path)
-            oi[i] = own ? (int64_t)ri : -1;
-            od[i] = own ? decode<MODE>(rk, a.metric) : __builtin_huge_val();
+#pragma unroll
+        for (int s = 0; s < NR; ++s) {  // lane i of slot s holds R[i + 64 s]; kstride <= 64 NR
+            const int i = lane + 64 * s;
+            if (i < a.kstride) {
+                const bool own = i < take;
+                oi[i] = own ? (int64_t)ri[s] : -1;
+                od[i] = own ? decode<MODE>(rk[s], a.metric) : __builtin_huge_val();
+            }
         }
     } else {
         int rb = 0;
@@ -1503,7 +1560,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
 // Register-beam kernel: at least four waves per SIMD (<= 128 VGPRs), six (<= 80) for the
 // constant shape, whose LDS fits 24 waves per CU.
 template <int MODE, int AUX, int SH>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SH == kShapeM16D128 ? 6 : 4))) void search_fast_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SH != kShapeAny ? 6 : 4))) void search_fast_kernel(
     SearchArgs a) {
     search_loop<MODE, false, true, AUX, SH>(a);
 }
@@ -1525,6 +1582,7 @@ template __global__ void search_kernel<kSeqF32, true>(SearchArgs);
 template __global__ void search_kernel<kSeqU8, false>(SearchArgs);
 template __global__ void search_fast_kernel<kExactU8, 0, kShapeAny>(SearchArgs);
 template __global__ void search_fast_kernel<kExactU8, 0, kShapeM16D128>(SearchArgs);
+template __global__ void search_fast_kernel<kExactU8, 0, kShapeM16D128Wide>(SearchArgs);
 template __global__ void search_fast_kernel<kSeqF32, 0, kShapeAny>(SearchArgs);
 template __global__ void search_fast_kernel<kSeqU8, 0, kShapeAny>(SearchArgs);
 template __global__ void search_fast_kernel<kExactU8, 1, kShapeAny>(SearchArgs);
@@ -1556,6 +1614,7 @@ static KernelFn pick(int mode, bool filt, const SearchArgs &a) {
                    : search_stop_kernel<kExactU8, kShapeAny>;
     }
     if (fast && !filt) {
+        if (a.nr == 2) return search_fast_kernel<kExactU8, 0, kShapeM16D128Wide>;
         if (mode == kExactU8)
             return m16 ? search_fast_kernel<kExactU8, 0, kShapeM16D128>
                        : search_fast_kernel<kExactU8, 0, kShapeAny>;

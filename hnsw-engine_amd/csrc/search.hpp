@@ -63,6 +63,7 @@ struct SearchArgs {
     int lds_per_wave;
     int slots;
     int fast;  // 1: unfiltered, every ef <= 64 -> the register-beam layer-0 kernel
+    int nr;    // register beam: R entries per lane (2: the constant shape with ef <= 128)
     // learned stopping (register-beam kernel only): model parameters (layout below) and the
     // logit above which a beam stops; at least stop_tmin layer-0 expansions run first
     const float *stop_w;
@@ -104,13 +105,13 @@ __host__ __device__ inline int visited_hash_bytes(int h_log2, int id_bits) {
 // with its flags (shared with the new-neighbour staging), 64-entry tie list and the visited
 // hash (layer 0 with 1 << h_log2 slots, upper layers with at most 512). 128 dimensions, 1M ids
 // and 2,048 slots: 6,016 bytes (24 waves per CU fit).
-inline int lds_bytes_fast(int mode, int dim, int dimp, int h_log2, int id_bits) {
+inline int lds_bytes_fast(int mode, int dim, int dimp, int h_log2, int id_bits, int nr = 1) {
     auto up = [](int v) { return (v + 15) & ~15; };
     int q = mode == kExactU8 ? up(dimp) : up(dim * 8);
     const int upper = h_log2 < 9 ? h_log2 : 9;
     const int h0 = visited_hash_bytes(h_log2, id_bits), hu = visited_hash_bytes(upper, id_bits);
     const int hash = h0 > hu ? h0 : hu;
-    return q + 2 * (64 * 8 + 64 * 4) + 64 * 4 + hash;
+    return q + nr * 64 * 8 + 64 * 4 + 64 * 8 + 64 * 4 + nr * 64 * 4 + hash;
 }
 
 // Bytes of LDS one wave needs for the given capacities.

@@ -62,11 +62,27 @@ def _oracle(eng, queries, k, ef):
                                queries, k, ef, threads=8)
 
 
-@pytest.mark.parametrize("ef,k", [(16, 10), (48, 10), (64, 64), (10, 1)])
+@pytest.mark.parametrize("ef,k", [(16, 10), (48, 10), (64, 64), (10, 1), (65, 10), (96, 10),
+                                  (128, 100), (100, 128)])
 def test_constant_shape_kernel_matches_oracle(m16d128, ef: int, k: int) -> None:
+    """ef <= 64: one R entry per lane; 65..128: two (the wide register beam)."""
     eng, _, queries = m16d128
     print(f"case m16d128 ef={ef} k={k}", flush=True)
     _same(eng.search_many(queries, k, ef), _oracle(eng, queries, k, ef), f"ef={ef}")
+
+
+def test_wide_beam_mixed_batch_matches_oracle(m16d128) -> None:
+    """Per-query ef in 16..128 and k in {1, 10, 100} in one launch of the wide beam."""
+    eng, _, queries = m16d128
+    rng = np.random.default_rng(17)
+    nq = queries.shape[0]
+    k = rng.choice([1, 10, 100], size=nq)
+    ef = np.maximum(rng.choice([16, 40, 64, 80, 128], size=nq), k)
+    print("case m16d128 mixed ef/k", flush=True)
+    got = eng.search_many(queries, k, ef)
+    want = oracle.search_batch(eng.native_arrays(), eng.entry_point, eng.max_level, "l2",
+                               queries, k, ef, threads=8)
+    _same(got, want, "mixed")
 
 
 @pytest.mark.parametrize("fast_hash_log2,probe_limit", [(0, 1), (0, 3), (6, 0), (8, 0), (12, 0)])
@@ -84,6 +100,22 @@ def test_visited_variants_match_oracle(m16d128, fast_hash_log2: int, probe_limit
     _same(got, want, f"hash_log2={fast_hash_log2} probe_limit={probe_limit}")
     if probe_limit == 1 or fast_hash_log2 == 6:
         assert moves > 0  # the bitset path ran
+
+
+def test_wide_beam_on_the_bitset_matches_oracle(m16d128) -> None:
+    """The two-entry beam with its visited set moved to the bitset after every probe step."""
+    eng, _, queries = m16d128
+    want = _oracle(eng, queries, 10, 96)
+    print("case m16d128 ef=96 probe_limit=1", flush=True)
+    _moves(eng)
+    _tune(eng, 0, 1)
+    try:
+        got = eng.search_many(queries, 10, 96)
+        moves = _moves(eng)
+    finally:
+        _tune(eng)
+    _same(got, want, "ef=96 probe_limit=1")
+    assert moves > 0
 
 
 def test_whole_id_slots_match_oracle(wide_ids) -> None:
