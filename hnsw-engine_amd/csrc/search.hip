@@ -1095,11 +1095,12 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                     th = 0;
                     tn = 0;
                 }
+                // (an R entry and a new entry of the same lane can both be such)
                 const bool er = lane < rn && pos_r >= ef && !rflag && rk == nwk;
                 const bool ea = acc && pos_a >= ef && ki == nwk;
-                const uint64_t em = ballot(er || ea);
-                if (em) {
-                    const int ne = popc(em);
+                const uint64_t emr = ballot(er), ema = ballot(ea);
+                if (emr | ema) {
+                    const int ne = popc(emr) + popc(ema);
                     // compact the live ties to the front, then append the new ones and sort
                     const int live_t = tn - th;
                     if (live_t + ne > kTies) {
@@ -1118,10 +1119,13 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                         ti[lane] = i0;
                     }
                     wave_sync();
-                    if (er || ea) {
-                        const int d = live_t + rank_below(em);
-                        tk[d] = er ? rk : ki;
-                        ti[d] = er ? ri : ii;
+                    if (er) {
+                        tk[live_t + rank_below(emr)] = rk;
+                        ti[live_t + rank_below(emr)] = ri;
+                    }
+                    if (ea) {
+                        tk[live_t + popc(emr) + rank_below(ema)] = ki;
+                        ti[live_t + popc(emr) + rank_below(ema)] = ii;
                     }
                     th = 0;
                     tn = live_t + ne;
