@@ -130,7 +130,8 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, 
     // ef 65..128 on the constant shape: the register beam with two entries per lane.
     const bool wide = allow_fast && !filt && !fast && !b.trace && mode == kExactU8 &&
                       b.max_ef <= 128 && b.kstride <= 128 && h->w0 == 32 && h->w1 == 16 &&
-                      h->dimp == 128;
+                      h->dimp == 128 &&
+                      a.id_bits - (h->tune_fast_hlog ? h->tune_fast_hlog : 11) <= 10;
     plan(a, mode, h, b.max_ef,
          fast || wide || (allow_fast && !filt && b.max_ef <= 64 && b.kstride <= 64),
          wide ? 2 : 1);

@@ -118,8 +118,14 @@ template <int SH> constexpr int nr_of() { return SH == kShapeM16D128Wide ? 2 : 1
 template <int SH> HM_DEV int w0_of(const SearchArgs &a) { return SH != kShapeAny ? 32 : a.w0; }
 template <int SH> HM_DEV int w1_of(const SearchArgs &a) { return SH != kShapeAny ? 16 : a.w1; }
 
+// Quotient visited slots (16 bits) are used when the ids have at most this many bits more than
+// the table index.
+constexpr int kQuotientMaxRemainder = 10;
+
+// The constant-shape kernels also assume quotient slots at layer 0.
 inline bool shape_m16d128(const SearchArgs &a, int mode) {
-    return mode == kExactU8 && a.w0 == 32 && a.w1 == 16 && a.dimp == 128;
+    return mode == kExactU8 && a.w0 == 32 && a.w1 == 16 && a.dimp == 128 &&
+           a.id_bits - a.h_log2 <= kQuotientMaxRemainder;
 }
 
 // Number of entries of sorted (k, i)[0..n) strictly below (bk, bi).
@@ -341,8 +347,6 @@ constexpr uint32_t inverse_odd(uint32_t a) {  // Newton iteration: 3, 6, 12, 24,
 constexpr uint32_t kHashMulInv = inverse_odd(kHashMul);
 static_assert(kHashMul * kHashMulInv == 1u, "kHashMul must be invertible modulo 2^32");
 
-// Quotient mode is used when the remainder leaves at least 6 bits of probe distance.
-constexpr int kQuotientMaxRemainder = 10;
 
 // Zeroes the bitset words this layer search set: through the id list when it held them all,
 // otherwise the whole bitset.
@@ -417,11 +421,13 @@ HM_DEV void visited_make_room(const Lds &L, Visited &V, int incoming) {
 }
 
 // True if `id` was not yet visited in this layer; marks it.
+HM_DEV bool visit_bits(const Visited &V, uint32_t id) {
+    const uint32_t bit = 1u << (id & 31);
+    return (atomicOr(&V.bits[id >> 5], bit) & bit) == 0u;
+}
+
 HM_DEV bool visit(const Lds &L, const Visited &V, uint32_t id) {
-    if (V.use_bits) {
-        uint32_t bit = 1u << (id & 31);
-        return (atomicOr(&V.bits[id >> 5], bit) & bit) == 0u;
-    }
+    if (V.use_bits) return visit_bits(V, id);
     const uint32_t v = id + 1u, mask = (1u << V.hlog) - 1u;
     uint32_t pos = (id * 2654435761u) >> (32 - V.hlog);
     for (;;) {
@@ -479,6 +485,13 @@ HM_DEV bool visit_quotient(const Lds &L, Visited &V, uint32_t id, bool active) {
 // every lane of the wave.
 HM_DEV bool visit_lanes(const Lds &L, Visited &V, uint32_t id, bool active) {
     if (!V.quotient || V.use_bits) return active && visit(L, V, id);
+    return visit_quotient(L, V, id, active);
+}
+
+// visit_lanes for a layer search known to use quotient slots (the constant-shape kernels at
+// layer 0): two forms instead of three keep the probe loop's control flow small.
+HM_DEV bool visit_lanes_quotient(const Lds &L, Visited &V, uint32_t id, bool active) {
+    if (V.use_bits) return active && visit_bits(V, id);
     return visit_quotient(L, V, id, active);
 }
 
@@ -1203,7 +1216,7 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
     uint64_t *const tk = L.ck[0];  // evicted ties (key, id), sorted, [th, tn)
     uint32_t *const ti = L.ci[0];
     visited_reset(a, L, V, a.h_log2);
-    visit_lanes(L, V, entry_id, lane == 0);
+    visit_lanes_quotient(L, V, entry_id, lane == 0);
     V.count = 1;
     wave_sync();
     uint64_t re[NR], expd[NR];
@@ -1244,7 +1257,7 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
         }
         const int32_t nb = lane < 32 ? a.links0[(size_t)cid * 32 + lane] : -1;
         visited_make_room(L, V, 32);
-        const bool isnew = visit_lanes(L, V, (uint32_t)nb, nb >= 0);
+        const bool isnew = visit_lanes_quotient(L, V, (uint32_t)nb, nb >= 0);
         visited_note(V, isnew, (uint32_t)nb);
         const uint64_t nm = ballot(isnew);
         const int nnew = popc(nm);
