@@ -320,10 +320,18 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
     return HM_OK;
 }
 
+// Branch-free over blocks so that the compiler vectorises it (NaN clamps to 0 and differs).
 bool all_u8_integers(const float *v, size_t count) {
-    for (size_t i = 0; i < count; ++i) {
-        float x = v[i];
-        if (!(x >= 0.0f && x <= 255.0f) || x != floorf(x)) return false;
+    constexpr size_t kBlock = 4096;
+    for (size_t i = 0; i < count; i += kBlock) {
+        const size_t e = std::min(count, i + kBlock);
+        int bad = 0;
+        for (size_t j = i; j < e; ++j) {
+            const float x = v[j];
+            const float c = std::min(std::max(x, 0.0f), 255.0f);
+            bad |= (int)(c != x) | (int)((float)(int)c != c);
+        }
+        if (bad) return false;
     }
     return true;
 }

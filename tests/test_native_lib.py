@@ -53,3 +53,21 @@ def test_builder_reproduces_search_fixture_graphs(name: str) -> None:
     ref = GraphState(arrays["vectors"], arrays["levels"], entry, top,
                      *[tuple(x) for x in golden_io.csr_layers(fx)[1:]])
     assert diff_states(eng.export_state(), ref) == []
+
+
+def test_exact_query_check_accepts_only_8bit_integers() -> None:
+    """hm_queries_exact decides whether a batch takes the exact 8-bit kernels (host code only):
+    every value must be an integer in 0..255 (-0.0 counts as 0)."""
+    lib = _native.lib()
+    q = np.random.default_rng(0).integers(0, 256, size=(300, 37)).astype(np.float32)
+    q[0, 0], q[-1, -1] = 0.0, 255.0
+    assert lib.hm_queries_exact(_native.ptr(q), q.size) == 1
+    for bad in (np.nan, np.inf, -np.inf, -1.0, 255.5, 256.0, 3.25, 1e-30):
+        for pos in ((0, 0), (150, 20), (299, 36)):
+            q2 = q.copy()
+            q2[pos] = bad
+            assert lib.hm_queries_exact(_native.ptr(q2), q2.size) == 0, (bad, pos)
+    q3 = q.copy()
+    q3[7, 7] = -0.0
+    assert lib.hm_queries_exact(_native.ptr(q3), q3.size) == 1
+    assert lib.hm_queries_exact(_native.ptr(q), 0) == 1
