@@ -54,13 +54,13 @@ HM_DEV int shfl_xor(int v, int m) { return __shfl_xor(v, m, kWave); }
 // or not by a select instead of a branch; the 32- and 64-lane steps cross rows through
 // ds_bpermute. All lanes must be active.
 HM_DEV int group_sum(int v, int g) {
-    int t = __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);
+    int t = __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true);
     v += g >= 2 ? t : 0;
-    t = __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);
+    t = __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true);
     v += g >= 4 ? t : 0;
-    t = __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);
+    t = __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, true);
     v += g >= 8 ? t : 0;
-    t = __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false);
+    t = __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, true);
     v += g >= 16 ? t : 0;
     if (g >= 32) {
         v += shfl_xor(v, 16);
@@ -74,13 +74,13 @@ HM_DEV int group_sum(int v, int g) {
 HM_DEV int group_sum_f32(float x) {
     int v = __float_as_int(x);
     v = __float_as_int(__int_as_float(v) +
-                       __int_as_float(__builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false)));
+                       __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true)));
     v = __float_as_int(__int_as_float(v) +
-                       __int_as_float(__builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false)));
+                       __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true)));
     v = __float_as_int(__int_as_float(v) +
-                       __int_as_float(__builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false)));
+                       __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, true)));
     v = __float_as_int(__int_as_float(v) +
-                       __int_as_float(__builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false)));
+                       __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, true)));
     return v;
 }
 
