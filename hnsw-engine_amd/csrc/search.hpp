@@ -62,7 +62,7 @@ struct SearchArgs {
     int r_cap, c_cap, h_log2;
     int lds_per_wave;
     int slots;
-    int fast;  // 1: unfiltered, every ef <= 64 -> the register-beam layer-0 kernel
+    int fast;  // 1: unfiltered, every ef <= 64 (<= 128 when nr = 2) -> the register beam
     int nr;    // register beam: R entries per lane (2: the constant shape with ef <= 128)
     // learned stopping (register-beam kernel only): model parameters (layout below) and the
     // logit above which a beam stops; at least stop_tmin layer-0 expansions run first
@@ -101,7 +101,7 @@ __host__ __device__ inline int visited_hash_bytes(int h_log2, int id_bits) {
     return (id_bits - h_log2 <= 10 ? 2 : 4) << h_log2;
 }
 
-// Bytes of LDS one wave of the register-beam kernel needs: query, 64-entry R scatter buffer
+// Bytes of LDS one wave of the register-beam kernel needs: query, (64 nr)-entry R scatter buffer
 // with its flags (shared with the new-neighbour staging), 64-entry tie list and the visited
 // hash (layer 0 with 1 << h_log2 slots, upper layers with at most 512). 128 dimensions, 1M ids
 // and 2,048 slots: 6,016 bytes (24 waves per CU fit).
