@@ -75,8 +75,9 @@ void plan(SearchArgs &a, int mode, const hm_index *h, int max_ef, bool fast, int
         // at ef 48 outgrow it and continue on the global bitset; 1024 slots overflow for nearly
         // every query. The two-slot beam (ef 65..128) too: at ef 96 and 128 4096 slots avoid
         // nearly all moves to the bitset but leave 15 waves per CU instead of 23, and measured
-        // 2 % slower.
-        a.h_log2 = h->tune_fast_hlog ? h->tune_fast_hlog : (max_ef <= 8 ? 10 : 11);
+        // 2 % slower. The four-slot beam (ef 129..256, ~1,250 visits) takes 4096 slots.
+        a.h_log2 = h->tune_fast_hlog ? h->tune_fast_hlog
+                                     : (nr == 4 ? 12 : (max_ef <= 8 ? 10 : 11));
         a.r_cap = 64 * nr;
         a.c_cap = 64;
         a.lds_per_wave = lds_bytes_fast(mode, h->dim, h->dimp, a.h_log2, a.id_bits, nr);
@@ -127,14 +128,16 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, 
     a.out_hops = b.hops;
     a.out_fail = b.fail;
     const bool fast = allow_fast && !filt && b.max_ef <= 64 && b.kstride <= 64 && mode != kSeqU8;
-    // ef 65..128 on the constant shape: the register beam with two entries per lane.
+    // ef 65..256 on the constant shape: the register beam with two or four entries per lane.
+    const int wide_nr = b.max_ef <= 128 && b.kstride <= 128 ? 2
+                        : (b.max_ef <= 256 && b.kstride <= 256 ? 4 : 0);
+    const int wide_hlog = h->tune_fast_hlog ? h->tune_fast_hlog : (wide_nr == 4 ? 12 : 11);
     const bool wide = allow_fast && !filt && !fast && !b.trace && mode == kExactU8 &&
-                      b.max_ef <= 128 && b.kstride <= 128 && h->w0 == 32 && h->w1 == 16 &&
-                      h->dimp == 128 &&
-                      a.id_bits - (h->tune_fast_hlog ? h->tune_fast_hlog : 11) <= 10;
+                      wide_nr > 0 && h->w0 == 32 && h->w1 == 16 && h->dimp == 128 &&
+                      a.id_bits - wide_hlog <= 10;
     plan(a, mode, h, b.max_ef,
          fast || wide || (allow_fast && !filt && b.max_ef <= 64 && b.kstride <= 64),
-         wide ? 2 : 1);
+         wide ? wide_nr : 1);
     if (fast && b.trace) {
         a.trace = b.trace;
         a.trace_steps = b.trace_steps;

@@ -112,9 +112,21 @@ HM_DEV Lds carve_fast(uint8_t *base, const SearchArgs &a, int mode, bool model) 
 // one-byte dimensions), where every loop bound and stride is a constant.
 constexpr int kShapeAny = 0;
 constexpr int kShapeM16D128 = 1;
-constexpr int kShapeM16D128Wide = 2;  // the same shape with two R entries per lane (ef <= 128)
+constexpr int kShapeM16D128Wide = 2;   // the same shape with two R entries per lane (ef <= 128)
+constexpr int kShapeM16D128Wide4 = 3;  // four R entries per lane (ef <= 256)
 
-template <int SH> constexpr int nr_of() { return SH == kShapeM16D128Wide ? 2 : 1; }
+template <int SH> constexpr int nr_of() {
+    return SH == kShapeM16D128Wide ? 2 : (SH == kShapeM16D128Wide4 ? 4 : 1);
+}
+
+// v[s] for a wave-uniform slot index s of a register array (a select chain, no scratch).
+template <int NR, typename T> HM_DEV T slot_of(const T (&v)[NR], int s) {
+    T r = v[0];
+#pragma unroll
+    for (int i = 1; i < NR; ++i)
+        if (s == i) r = v[i];
+    return r;
+}
 template <int SH> HM_DEV int w0_of(const SearchArgs &a) { return SH != kShapeAny ? 32 : a.w0; }
 template <int SH> HM_DEV int w1_of(const SearchArgs &a) { return SH != kShapeAny ? 16 : a.w1; }
 
@@ -1279,7 +1291,7 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
 #pragma unroll
             for (int s = 0; s < NR; ++s) rkey[s] = key_hi(re[s]);
             const uint64_t all_new = (1ull << nnew) - 1ull;  // nnew <= 32
-            const uint32_t worst = readlane_u32(rkey[(ef - 1) >> 6], (ef - 1) & 63);
+            const uint32_t worst = readlane_u32(slot_of<NR>(rkey, (ef - 1) >> 6), (ef - 1) & 63);
             const uint64_t part = rn == ef ? ballot(lane < nnew && nk < worst) : all_new;
             int c_r = 0, c_n = 0;
             for (uint64_t m = part; m; m &= m - 1ull) {
@@ -1339,7 +1351,8 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
             if (total > ef) {
                 // Evictions: keep the unexpanded ones whose key equals the new worst (an R entry
                 // and a new one of the same lane can both be such).
-                const uint32_t nwk = key_hi(readlane_u64(nre[(ef - 1) >> 6], (ef - 1) & 63));
+                const uint32_t nwk =
+                    key_hi(readlane_u64(slot_of<NR>(nre, (ef - 1) >> 6), (ef - 1) & 63));
                 const bool was_full = rn == ef;
                 if (!was_full || nwk < worst) {
                     th = 0;
@@ -1575,9 +1588,10 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
 }
 
 // Register-beam kernel: at least four waves per SIMD (<= 128 VGPRs), six (<= 80) for the
-// constant shape, whose LDS fits 24 waves per CU.
+// constant shape with one or two entries per lane, whose LDS fits 24 or 23 waves per CU (four
+// entries per lane: 12 KB of LDS, 13 waves per CU, so four per SIMD and no spills).
 template <int MODE, int AUX, int SH>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SH != kShapeAny ? 6 : 4))) void search_fast_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SH == kShapeM16D128 || SH == kShapeM16D128Wide ? 6 : 4))) void search_fast_kernel(
     SearchArgs a) {
     search_loop<MODE, false, true, AUX, SH>(a);
 }
@@ -1600,6 +1614,7 @@ template __global__ void search_kernel<kSeqU8, false>(SearchArgs);
 template __global__ void search_fast_kernel<kExactU8, 0, kShapeAny>(SearchArgs);
 template __global__ void search_fast_kernel<kExactU8, 0, kShapeM16D128>(SearchArgs);
 template __global__ void search_fast_kernel<kExactU8, 0, kShapeM16D128Wide>(SearchArgs);
+template __global__ void search_fast_kernel<kExactU8, 0, kShapeM16D128Wide4>(SearchArgs);
 template __global__ void search_fast_kernel<kSeqF32, 0, kShapeAny>(SearchArgs);
 template __global__ void search_fast_kernel<kSeqU8, 0, kShapeAny>(SearchArgs);
 template __global__ void search_fast_kernel<kExactU8, 1, kShapeAny>(SearchArgs);
@@ -1632,6 +1647,7 @@ static KernelFn pick(int mode, bool filt, const SearchArgs &a) {
     }
     if (fast && !filt) {
         if (a.nr == 2) return search_fast_kernel<kExactU8, 0, kShapeM16D128Wide>;
+        if (a.nr == 4) return search_fast_kernel<kExactU8, 0, kShapeM16D128Wide4>;
         if (mode == kExactU8)
             return m16 ? search_fast_kernel<kExactU8, 0, kShapeM16D128>
                        : search_fast_kernel<kExactU8, 0, kShapeAny>;

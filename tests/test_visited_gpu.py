@@ -63,9 +63,9 @@ def _oracle(eng, queries, k, ef):
 
 
 @pytest.mark.parametrize("ef,k", [(16, 10), (48, 10), (64, 64), (10, 1), (65, 10), (96, 10),
-                                  (128, 100), (100, 128)])
+                                  (128, 100), (100, 128), (129, 10), (200, 100), (256, 256)])
 def test_constant_shape_kernel_matches_oracle(m16d128, ef: int, k: int) -> None:
-    """ef <= 64: one R entry per lane; 65..128: two (the wide register beam)."""
+    """ef <= 64: one R entry per lane; 65..128: two; 129..256: four (the wide register beams)."""
     eng, _, queries = m16d128
     print(f"case m16d128 ef={ef} k={k}", flush=True)
     _same(eng.search_many(queries, k, ef), _oracle(eng, queries, k, ef), f"ef={ef}")
@@ -100,6 +100,20 @@ def test_visited_variants_match_oracle(m16d128, fast_hash_log2: int, probe_limit
     _same(got, want, f"hash_log2={fast_hash_log2} probe_limit={probe_limit}")
     if probe_limit == 1 or fast_hash_log2 == 6:
         assert moves > 0  # the bitset path ran
+
+
+def test_four_entry_beam_mixed_batch_matches_oracle(m16d128) -> None:
+    """Per-query ef in 16..256 and k in {1, 10, 100} in one launch of the four-entry beam."""
+    eng, _, queries = m16d128
+    rng = np.random.default_rng(19)
+    nq = queries.shape[0]
+    k = rng.choice([1, 10, 100], size=nq)
+    ef = np.maximum(rng.choice([16, 64, 128, 192, 256], size=nq), k)
+    print("case m16d128 mixed ef/k up to 256", flush=True)
+    got = eng.search_many(queries, k, ef)
+    want = oracle.search_batch(eng.native_arrays(), eng.entry_point, eng.max_level, "l2",
+                               queries, k, ef, threads=8)
+    _same(got, want, "mixed 256")
 
 
 def test_wide_beam_on_the_bitset_matches_oracle(m16d128) -> None:
