@@ -75,9 +75,10 @@ void plan(SearchArgs &a, int mode, const hm_index *h, int max_ef, bool fast, int
         // at ef 48 outgrow it and continue on the global bitset; 1024 slots overflow for nearly
         // every query. The two-slot beam (ef 65..128) too: at ef 96 and 128 4096 slots avoid
         // nearly all moves to the bitset but leave 15 waves per CU instead of 23, and measured
-        // 2 % slower. The four-slot beam (ef 129..256, ~1,250 visits) takes 4096 slots.
+        // 2 % slower. The four- and eight-slot beams (ef 129..512, 1,250-1,400 visits) take
+        // 4096 slots.
         a.h_log2 = h->tune_fast_hlog ? h->tune_fast_hlog
-                                     : (nr == 4 ? 12 : (max_ef <= 8 ? 10 : 11));
+                                     : (nr >= 4 ? 12 : (max_ef <= 8 ? 10 : 11));
         a.r_cap = 64 * nr;
         a.c_cap = 64;
         a.lds_per_wave = lds_bytes_fast(mode, h->dim, h->dimp, a.h_log2, a.id_bits, nr);
@@ -128,10 +129,12 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, 
     a.out_hops = b.hops;
     a.out_fail = b.fail;
     const bool fast = allow_fast && !filt && b.max_ef <= 64 && b.kstride <= 64 && mode != kSeqU8;
-    // ef 65..256 on the constant shape: the register beam with two or four entries per lane.
-    const int wide_nr = b.max_ef <= 128 && b.kstride <= 128 ? 2
-                        : (b.max_ef <= 256 && b.kstride <= 256 ? 4 : 0);
-    const int wide_hlog = h->tune_fast_hlog ? h->tune_fast_hlog : (wide_nr == 4 ? 12 : 11);
+    // ef 65..512 on the constant shape: the register beam with two, four or eight entries per
+    // lane.
+    const int wide_nr = b.max_ef <= 128 && b.kstride <= 128   ? 2
+                        : b.max_ef <= 256 && b.kstride <= 256 ? 4
+                        : b.max_ef <= 512 && b.kstride <= 512 ? 8 : 0;
+    const int wide_hlog = h->tune_fast_hlog ? h->tune_fast_hlog : (wide_nr >= 4 ? 12 : 11);
     const bool wide = allow_fast && !filt && !fast && !b.trace && mode == kExactU8 &&
                       wide_nr > 0 && h->w0 == 32 && h->w1 == 16 && h->dimp == 128 &&
                       a.id_bits - wide_hlog <= 10;

@@ -114,9 +114,12 @@ constexpr int kShapeAny = 0;
 constexpr int kShapeM16D128 = 1;
 constexpr int kShapeM16D128Wide = 2;   // the same shape with two R entries per lane (ef <= 128)
 constexpr int kShapeM16D128Wide4 = 3;  // four R entries per lane (ef <= 256)
+constexpr int kShapeM16D128Wide8 = 4;  // eight R entries per lane (ef <= 512)
 
 template <int SH> constexpr int nr_of() {
-    return SH == kShapeM16D128Wide ? 2 : (SH == kShapeM16D128Wide4 ? 4 : 1);
+    return SH == kShapeM16D128Wide ? 2
+           : SH == kShapeM16D128Wide4 ? 4
+           : SH == kShapeM16D128Wide8 ? 8 : 1;
 }
 
 // v[s] for a wave-uniform slot index s of a register array (a select chain, no scratch).
@@ -1615,6 +1618,7 @@ template __global__ void search_fast_kernel<kExactU8, 0, kShapeAny>(SearchArgs);
 template __global__ void search_fast_kernel<kExactU8, 0, kShapeM16D128>(SearchArgs);
 template __global__ void search_fast_kernel<kExactU8, 0, kShapeM16D128Wide>(SearchArgs);
 template __global__ void search_fast_kernel<kExactU8, 0, kShapeM16D128Wide4>(SearchArgs);
+template __global__ void search_fast_kernel<kExactU8, 0, kShapeM16D128Wide8>(SearchArgs);
 template __global__ void search_fast_kernel<kSeqF32, 0, kShapeAny>(SearchArgs);
 template __global__ void search_fast_kernel<kSeqU8, 0, kShapeAny>(SearchArgs);
 template __global__ void search_fast_kernel<kExactU8, 1, kShapeAny>(SearchArgs);
@@ -1648,6 +1652,7 @@ static KernelFn pick(int mode, bool filt, const SearchArgs &a) {
     if (fast && !filt) {
         if (a.nr == 2) return search_fast_kernel<kExactU8, 0, kShapeM16D128Wide>;
         if (a.nr == 4) return search_fast_kernel<kExactU8, 0, kShapeM16D128Wide4>;
+        if (a.nr == 8) return search_fast_kernel<kExactU8, 0, kShapeM16D128Wide8>;
         if (mode == kExactU8)
             return m16 ? search_fast_kernel<kExactU8, 0, kShapeM16D128>
                        : search_fast_kernel<kExactU8, 0, kShapeAny>;

@@ -63,9 +63,11 @@ def _oracle(eng, queries, k, ef):
 
 
 @pytest.mark.parametrize("ef,k", [(16, 10), (48, 10), (64, 64), (10, 1), (65, 10), (96, 10),
-                                  (128, 100), (100, 128), (129, 10), (200, 100), (256, 256)])
+                                  (128, 100), (100, 128), (129, 10), (200, 100), (256, 256),
+                                  (257, 10), (512, 100)])
 def test_constant_shape_kernel_matches_oracle(m16d128, ef: int, k: int) -> None:
-    """ef <= 64: one R entry per lane; 65..128: two; 129..256: four (the wide register beams)."""
+    """ef <= 64: one R entry per lane; 65..128: two; 129..256: four; 257..512: eight (the wide
+    register beams)."""
     eng, _, queries = m16d128
     print(f"case m16d128 ef={ef} k={k}", flush=True)
     _same(eng.search_many(queries, k, ef), _oracle(eng, queries, k, ef), f"ef={ef}")
