@@ -22,6 +22,10 @@ struct hm_index {
     hm::DevBuf<uint64_t> spill_key;
     hm::DevBuf<uint32_t> spill_id;
     hm::DevBuf<int32_t> queue;
+    // two-phase schedule: per query layer-0 entry, counters after the descent, and the order
+    hm::DevBuf<uint64_t> pre_key;
+    hm::DevBuf<uint32_t> pre_id;
+    hm::DevBuf<int32_t> pre_dc, pre_hops, order;
     // staging of the host-buffer entry points
     hm::DevBuf<float> q32;
     hm::DevBuf<double> q64;
@@ -37,11 +41,13 @@ struct hm_index {
     // tuning (0: automatic): visited-hash size of the register-beam and general kernels, waves
     // per compute unit
     int tune_fast_hlog = 0, tune_hlog = 0, tune_waves_per_cu = 0, tune_probe_limit = 0;
+    int tune_schedule = 0;  // 1: one launch per batch (no longest-first two-phase schedule)
     hm::DevBuf<int32_t> stats;  // event counters, see SearchArgs::stats
 
     void release_all() {
-        hm::DevBuf<int32_t> *i32[] = {&links0, &links_up, &upper, &queue, &kq,
-                                      &efq,    &midx,     &dc,    &hops,  &fail};
+        hm::DevBuf<int32_t> *i32[] = {&links0, &links_up, &upper, &queue,    &kq,   &efq,
+                                      &midx,   &dc,       &hops,  &fail,     &pre_dc,
+                                      &pre_hops, &order};
         for (auto *b : i32) b->release();
         vec.release();
         u8.release();
@@ -49,6 +55,8 @@ struct hm_index {
         bits.release();
         vlist.release();
         spill_key.release();
+        pre_key.release();
+        pre_id.release();
         spill_id.release();
         q32.release();
         q64.release();

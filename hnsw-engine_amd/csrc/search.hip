@@ -397,8 +397,8 @@ HM_DEV void visited_reset(const SearchArgs &a, const Lds &L, Visited &V, int hlo
     V.count = 0;
     V.rbits = V.id_bits > hlog ? V.id_bits - hlog : 0;
     V.quotient = V.rbits <= kQuotientMaxRemainder;
-    const int words = V.quotient ? (1 << hlog) / 2 : (1 << hlog);
-    for (int i = lane; i < words; i += 64) L.hash[i] = 0u;
+    const int words = V.quotient ? (1 << hlog) / 2 : (1 << hlog);  // a multiple of 4 (hlog >= 6)
+    for (int i = 4 * lane; i < words; i += 256) *(uint4 *)&L.hash[i] = make_uint4(0, 0, 0, 0);
     wave_sync();
 }
 
@@ -554,12 +554,24 @@ HM_DEV void greedy_layer(const SearchArgs &a, const Lds &L, Visited &V, Query &Q
             V.count += nnew;
             uint64_t key = distances<MODE, SH>(a, L, nnew, Q.qq);
             Q.dcount += nnew;
-            uint32_t idx = lane < nnew ? (uint32_t)lane : kInfId;
-            wave_min_pair(key, idx);
-            if (key < cur_k) {
-                cur_k = key;
-                cur_id = L.nbi[idx];
-                improved = true;
+            if constexpr (MODE == kExactU8) {
+                // Exact keys fit in 32 bits: the row minimum by DPP, its first lane by ballot.
+                const uint32_t k32 = lane < nnew ? (uint32_t)key : ~0u;
+                const uint32_t m = wave_min_u32(k32);
+                if ((uint64_t)m < cur_k) {
+                    const int idx = __ffsll((long long)ballot(lane < nnew && k32 == m)) - 1;
+                    cur_k = m;
+                    cur_id = L.nbi[idx];
+                    improved = true;
+                }
+            } else {
+                uint32_t idx = lane < nnew ? (uint32_t)lane : kInfId;
+                wave_min_pair(key, idx);
+                if (key < cur_k) {
+                    cur_k = key;
+                    cur_id = L.nbi[idx];
+                    improved = true;
+                }
             }
             wave_sync();
         }
@@ -1495,16 +1507,35 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
             qd[i] = MODE == kSeqU8 ? a.q_f64[(size_t)qi * a.dim + i]
                                    : (double)a.q_f32[(size_t)qi * a.dim + i];
     }
-    if (lane == 0) L.nbi[0] = (uint32_t)a.entry;
-    wave_sync();
-
-    // Entry point distance (numba_engine.py:449-451), then greedy descent.
-    uint64_t cur_k = shfl_u64(distances<MODE, SH>(a, L, 1, Q.qq), 0);
-    uint32_t cur_id = (uint32_t)a.entry;
-    Q.dcount = 1;
-    wave_sync();
-    for (int layer = a.max_level; layer > 0; --layer)
-        greedy_layer<MODE, SH>(a, L, V, Q, layer, cur_k, cur_id);
+    uint64_t cur_k;
+    uint32_t cur_id;
+    if (a.phase == 2) {  // the descent ran in phase 1
+        cur_k = a.pre_key[qi];
+        cur_id = a.pre_id[qi];
+        Q.dcount = a.pre_dc[qi];
+        Q.hops = a.pre_hops[qi];
+        wave_sync();
+    } else {
+        if (lane == 0) L.nbi[0] = (uint32_t)a.entry;
+        wave_sync();
+        // Entry point distance (numba_engine.py:449-451), then greedy descent.
+        cur_k = shfl_u64(distances<MODE, SH>(a, L, 1, Q.qq), 0);
+        cur_id = (uint32_t)a.entry;
+        Q.dcount = 1;
+        wave_sync();
+        for (int layer = a.max_level; layer > 0; --layer)
+            greedy_layer<MODE, SH>(a, L, V, Q, layer, cur_k, cur_id);
+        if (a.phase == 1) {
+            if (lane == 0) {
+                a.pre_key[qi] = cur_k;
+                a.pre_id[qi] = cur_id;
+                a.pre_dc[qi] = Q.dcount;
+                a.pre_hops[qi] = Q.hops;
+            }
+            wave_sync();
+            return;
+        }
+    }
 
     int64_t *oi = a.out_ids + (size_t)qi * a.kstride;
     double *od = a.out_d + (size_t)qi * a.kstride;
@@ -1578,6 +1609,7 @@ HM_DEV void search_loop(const SearchArgs &a) {
         if (lane_id() == 0) qi = atomicAdd(a.queue, 1);
         qi = uniform(shfl(qi, 0));
         if (qi >= a.nq) break;
+        if (a.order) qi = uniform(a.order[qi]);
         run_query<MODE, FILT, FAST, AUX, SH>(a, L, V, qi, slot, M);
     }
     // Leave the bitset clean for the next launch.
@@ -1608,6 +1640,36 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SH == kShape
 }
 
 __global__ void reset_queue_kernel(int32_t *q) { q[0] = 0; }
+
+// Orders queries by their layer-0 entry key, largest first (a counting sort on 256 buckets of
+// the key's log2 with 3 fraction bits; order within a bucket is arbitrary): the farther a query
+// enters layer 0 from its nearest neighbours, the more expansions it tends to take, and taking
+// long queries first shortens a launch's tail. One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void order_by_entry_kernel(const uint64_t *key, int nq,
+                                                              int32_t *order) {
+    __shared__ int32_t count[256];
+    __shared__ int32_t start[256];
+    const int t = threadIdx.x;
+    if (t < 256) count[t] = 0;
+    __syncthreads();
+    auto bucket = [](uint64_t k) {
+        const uint32_t v = (uint32_t)min<uint64_t>(k, 0xFFFFFFFEull) + 1u;  // >= 1
+        const int e = 31 - __clz(v);
+        const int frac = e >= 3 ? (int)((v >> (e - 3)) & 7u) : (int)((v << (3 - e)) & 7u);
+        return 255 - min(255, e * 8 + frac);  // descending key -> ascending bucket
+    };
+    for (int i = t; i < nq; i += 1024) atomicAdd(&count[bucket(key[i])], 1);
+    __syncthreads();
+    if (t == 0) {
+        int s = 0;
+        for (int b = 0; b < 256; ++b) {
+            start[b] = s;
+            s += count[b];
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < nq; i += 1024) order[atomicAdd(&start[bucket(key[i])], 1)] = i;
+}
 
 template __global__ void search_kernel<kExactU8, false>(SearchArgs);
 template __global__ void search_kernel<kExactU8, true>(SearchArgs);
@@ -1685,8 +1747,23 @@ hipError_t search_launch(int mode, bool filt, const SearchArgs &a, int blocks, h
                                            a.lds_per_wave);
         if (e != hipSuccess) return e;
     }
+    if (a.phase == 0) {
+        hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, s, a.queue);
+        hipLaunchKernelGGL(fn, dim3(blocks), dim3(64), a.lds_per_wave, s, a);
+        return hipGetLastError();
+    }
+    // Two phases: every descent, the order, then layer 0 longest first.
+    SearchArgs d = a;
+    d.phase = 1;
+    d.order = nullptr;
     hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, s, a.queue);
-    hipLaunchKernelGGL(fn, dim3(blocks), dim3(64), a.lds_per_wave, s, a);
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(64), a.lds_per_wave, s, d);
+    hipLaunchKernelGGL(order_by_entry_kernel, dim3(1), dim3(1024), 0, s, a.pre_key, a.nq,
+                       const_cast<int32_t *>(a.order));
+    SearchArgs l = a;
+    l.phase = 2;
+    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, s, a.queue);
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(64), a.lds_per_wave, s, l);
     return hipGetLastError();
 }
 

@@ -48,6 +48,14 @@ struct SearchArgs {
     int32_t *out_dc;
     int32_t *out_hops;
     int32_t *out_fail;  // 1: candidate spill overflowed, rerun with a larger spill
+    // Two-phase schedule of the register beam: phase 1 runs each query's upper-layer descent
+    // and stores its layer-0 entry (key, id) and counters; phase 2 starts layer 0 from them,
+    // taking queries in `order` (longest expected first). phase 0: one launch does both.
+    int phase;
+    const int32_t *order;  // queue position -> query index (null: identity)
+    uint64_t *pre_key;
+    uint32_t *pre_id;
+    int32_t *pre_dc, *pre_hops;
     // persistent work queue and per-wave scratch (slot = global wave index)
     int32_t *queue;
     uint32_t *bitsets;

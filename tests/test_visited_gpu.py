@@ -25,8 +25,9 @@ def _same(got, want, msg=""):
         np.testing.assert_array_equal(g, w, err_msg=f"{msg} {name}")
 
 
-def _tune(eng, fast_hash_log2=0, probe_limit=0) -> None:
-    _native.check(_native.lib().hm_set_tuning(eng._device(), fast_hash_log2, 0, 0, probe_limit))
+def _tune(eng, fast_hash_log2=0, probe_limit=0, waves_per_cu=0, one_launch=0) -> None:
+    _native.check(_native.lib().hm_set_tuning(eng._device(), fast_hash_log2, 0, waves_per_cu,
+                                              probe_limit, one_launch))
 
 
 def _moves(eng) -> int:
@@ -116,6 +117,25 @@ def test_four_entry_beam_mixed_batch_matches_oracle(m16d128) -> None:
     want = oracle.search_batch(eng.native_arrays(), eng.entry_point, eng.max_level, "l2",
                                queries, k, ef, threads=8)
     _same(got, want, "mixed 256")
+
+
+@pytest.mark.parametrize("ef,k", [(48, 10), (96, 100), (300, 10)])
+def test_two_phase_schedule_matches_oracle(m16d128, ef: int, k: int) -> None:
+    """With fewer wave slots than queries (one wave per CU) a batch on the multi-entry register
+    beams (ef > 64) runs its descents first and layer 0 longest-first; one launch gives the same
+    results (ef = 48 runs in one launch either way)."""
+    eng, _, queries = m16d128
+    want = _oracle(eng, queries, k, ef)
+    print(f"case m16d128 two-phase ef={ef} k={k}", flush=True)
+    try:
+        _tune(eng, waves_per_cu=1)
+        two = eng.search_many(queries, k, ef)
+        _tune(eng, waves_per_cu=1, one_launch=1)
+        one = eng.search_many(queries, k, ef)
+    finally:
+        _tune(eng)
+    _same(two, want, f"two-phase ef={ef}")
+    _same(one, want, f"one launch ef={ef}")
 
 
 def test_wide_beam_on_the_bitset_matches_oracle(m16d128) -> None:
