@@ -527,9 +527,11 @@ HM_DEV bool allowed(const Query &Q, uint32_t id) {
 // the single best node; a neighbour is accepted iff strictly closer than the current best, so
 // the candidate after an expansion is the first occurrence of the row's minimum when that beats
 // the current best, and the search ends after the first expansion that does not improve.
+// uoff is cur_id's first upper row: each expansion reads it for every new neighbour alongside
+// their vectors, so the next node's row needs no dependent load.
 template <int MODE, int SH>
 HM_DEV void greedy_layer(const SearchArgs &a, const Lds &L, Visited &V, Query &Q, int layer,
-                         uint64_t &cur_k, uint32_t &cur_id) {
+                         uint64_t &cur_k, uint32_t &cur_id, int32_t &uoff) {
     const int lane = lane_id();
     visited_reset(a, L, V, a.h_log2 < 9 ? a.h_log2 : 9);
     visit_lanes(L, V, cur_id, lane == 0);
@@ -537,7 +539,7 @@ HM_DEV void greedy_layer(const SearchArgs &a, const Lds &L, Visited &V, Query &Q
     wave_sync();
     for (;;) {
         Q.hops += 1;
-        const int64_t row = (int64_t)a.upper_off[cur_id] + layer - 1;
+        const int64_t row = (int64_t)uoff + layer - 1;
         bool improved = false;
         const int w1 = w1_of<SH>(a);
         for (int base = 0; base < w1; base += 64) {
@@ -552,6 +554,7 @@ HM_DEV void greedy_layer(const SearchArgs &a, const Lds &L, Visited &V, Query &Q
             if (isnew) L.nbi[rank_below(m)] = (uint32_t)nb;
             wave_sync();
             V.count += nnew;
+            const int32_t nuo = lane < nnew ? a.upper_off[L.nbi[lane]] : 0;
             uint64_t key = distances<MODE, SH>(a, L, nnew, Q.qq);
             Q.dcount += nnew;
             if constexpr (MODE == kExactU8) {
@@ -562,6 +565,7 @@ HM_DEV void greedy_layer(const SearchArgs &a, const Lds &L, Visited &V, Query &Q
                     const int idx = __ffsll((long long)ballot(lane < nnew && k32 == m)) - 1;
                     cur_k = m;
                     cur_id = L.nbi[idx];
+                    uoff = __builtin_amdgcn_readlane(nuo, idx);
                     improved = true;
                 }
             } else {
@@ -570,6 +574,7 @@ HM_DEV void greedy_layer(const SearchArgs &a, const Lds &L, Visited &V, Query &Q
                 if (key < cur_k) {
                     cur_k = key;
                     cur_id = L.nbi[idx];
+                    uoff = __builtin_amdgcn_readlane(nuo, (int)idx);
                     improved = true;
                 }
             }
@@ -1523,8 +1528,9 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
         cur_id = (uint32_t)a.entry;
         Q.dcount = 1;
         wave_sync();
+        int32_t uoff = a.max_level > 0 ? a.upper_off[cur_id] : 0;
         for (int layer = a.max_level; layer > 0; --layer)
-            greedy_layer<MODE, SH>(a, L, V, Q, layer, cur_k, cur_id);
+            greedy_layer<MODE, SH>(a, L, V, Q, layer, cur_k, cur_id, uoff);
         if (a.phase == 1) {
             if (lane == 0) {
                 a.pre_key[qi] = cur_k;
