@@ -12,8 +12,9 @@ the stop MLP is trained on the GPU (<= 200,000 rows), and a sweep of (ef, thresh
 fastest setting with recall@10 >= 0.95. Then W untimed and K timed steps of the faster variant
 (the learned stop, unless it cannot reach the recall), a step being one batch search of all
 10,000 queries with queries and outputs already in HBM. The JSON line carries the roofline of
-the search kernel (algorithmic bytes per launch over the launch's average duration, HIP events
-on the launch stream) and the CPU baseline (the C oracle at the plain beam's ef, 1 thread and all
+one batch search (its kernels: the batched upper-layer descent, the longest-first order, the
+layer-0 beam; algorithmic bytes per batch over the batch's average duration, HIP events on the
+launch stream) and the CPU baseline (the C oracle at the plain beam's ef, 1 thread and all
 cores, on a bounded query sample, ids checked equal to the GPU's plain results).
 """
 
@@ -276,7 +277,8 @@ def main() -> int:
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "search_kernel", "launch_ms": launch_ms, "algo_bytes": algo_bytes,
+            "kernel": "one batch search: descend_kernel, order_by_entry_kernel, layer-0 kernel",
+            "launch_ms": launch_ms, "algo_bytes": algo_bytes,
         },
         "oracle": oracle_info,
         "cpu_baseline": cpu,

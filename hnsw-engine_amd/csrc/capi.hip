@@ -176,12 +176,14 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, 
     a.spill_id = h->spill_id.p;
     HM_TRY(h->queue.ensure(1), "queue allocation");
     a.queue = h->queue.p;
-    // More queries than wave slots on a register beam with two or more entries per lane
-    // (ef > 64): descents first, then layer 0 with the queries whose entries lie farthest from
-    // them first, which shortens the launch's tail (measured on 10,000 queries over 1M x 128:
-    // ef 96 -7 %, ef 256 -8 %). At ef <= 64 the separate descent launch costs more than the
-    // tail saves (ef 48 +7 %), so those batches run in one launch.
-    if (a.fast && a.nr >= 2 && !a.trace && !a.stop_w && b.nq > slots && h->tune_schedule == 0) {
+    // More queries than wave slots on the register beam (128 one-byte dimensions, 16 upper
+    // links): every query's upper-layer descent first, 16 lanes per query (descend_kernel), then
+    // layer 0 with the queries whose layer-0 entry lies farthest from them first, which
+    // shortens the launch's tail. Measured on 10,000 queries over 1M x 128 against one launch:
+    // ef 32 / 48 / 64 / 96 -12 / -17 / -16 / -14 %, ids identical.
+    const bool two_shape = mode == kExactU8 && h->w1 == 16 && h->dimp == 128 && h->dim <= 128;
+    if (a.fast && two_shape && !a.trace && b.nq > slots && h->tune_schedule == 0) {
+        a.descent_cap = h->tune_descent_cap > 0 ? h->tune_descent_cap : kDescentMaxCap;
         HM_TRY(h->pre_key.ensure(b.nq), "schedule allocation");
         HM_TRY(h->pre_id.ensure(b.nq), "schedule allocation");
         HM_TRY(h->pre_dc.ensure(b.nq), "schedule allocation");
@@ -604,13 +606,14 @@ int hm_search_codes(hm_index *h, const double *queries, const hm_search_opts *o,
 }
 
 int hm_set_tuning(hm_index *h, int32_t fast_hash_log2, int32_t hash_log2, int32_t waves_per_cu,
-                  int32_t probe_limit, int32_t one_launch) {
+                  int32_t probe_limit, int32_t one_launch, int32_t descent_cap) {
     if (!h) return invalid("null argument");
     if ((fast_hash_log2 && (fast_hash_log2 < 6 || fast_hash_log2 > 14)) ||
         (hash_log2 && (hash_log2 < 6 || hash_log2 > 14)) || waves_per_cu < 0 || probe_limit < 0 ||
-        one_launch < 0 || one_launch > 1)
+        one_launch < 0 || one_launch > 1 || descent_cap < 0 || descent_cap > kDescentMaxCap)
         return invalid("tuning values out of range");
     h->tune_schedule = one_launch;
+    h->tune_descent_cap = descent_cap;
     h->tune_fast_hlog = fast_hash_log2;
     h->tune_hlog = hash_log2;
     h->tune_waves_per_cu = waves_per_cu;

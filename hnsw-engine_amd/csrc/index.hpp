@@ -41,7 +41,8 @@ struct hm_index {
     // tuning (0: automatic): visited-hash size of the register-beam and general kernels, waves
     // per compute unit
     int tune_fast_hlog = 0, tune_hlog = 0, tune_waves_per_cu = 0, tune_probe_limit = 0;
-    int tune_schedule = 0;  // 1: one launch per batch (no longest-first two-phase schedule)
+    int tune_schedule = 0;  // 1: one launch per batch (no descents-first schedule)
+    int tune_descent_cap = 0;  // > 0: descend_kernel visited entries per layer (else the maximum)
     hm::DevBuf<int32_t> stats;  // event counters, see SearchArgs::stats
 
     void release_all() {

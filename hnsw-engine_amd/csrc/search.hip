@@ -1514,7 +1514,7 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
     }
     uint64_t cur_k;
     uint32_t cur_id;
-    if (a.phase == 2) {  // the descent ran in phase 1
+    if (a.phase == 2 && a.pre_hops[qi] >= 0) {  // descend_kernel ran the descent
         cur_k = a.pre_key[qi];
         cur_id = a.pre_id[qi];
         Q.dcount = a.pre_dc[qi];
@@ -1531,16 +1531,6 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
         int32_t uoff = a.max_level > 0 ? a.upper_off[cur_id] : 0;
         for (int layer = a.max_level; layer > 0; --layer)
             greedy_layer<MODE, SH>(a, L, V, Q, layer, cur_k, cur_id, uoff);
-        if (a.phase == 1) {
-            if (lane == 0) {
-                a.pre_key[qi] = cur_k;
-                a.pre_id[qi] = cur_id;
-                a.pre_dc[qi] = Q.dcount;
-                a.pre_hops[qi] = Q.hops;
-            }
-            wave_sync();
-            return;
-        }
     }
 
     int64_t *oi = a.out_ids + (size_t)qi * a.kstride;
@@ -1650,12 +1640,14 @@ __global__ void reset_queue_kernel(int32_t *q) { q[0] = 0; }
 // Orders queries by their layer-0 entry key, largest first (a counting sort on 256 buckets of
 // the key's log2 with 3 fraction bits; order within a bucket is arbitrary): the farther a query
 // enters layer 0 from its nearest neighbours, the more expansions it tends to take, and taking
-// long queries first shortens a launch's tail. One workgroup of 1024 threads.
+// long queries first shortens a launch's tail. One workgroup of 1024 threads; it also resets the
+// work queue of the layer-0 launch that follows.
 __global__ __launch_bounds__(1024) void order_by_entry_kernel(const uint64_t *key, int nq,
-                                                              int32_t *order) {
+                                                              int32_t *order, int32_t *queue) {
     __shared__ int32_t count[256];
     __shared__ int32_t start[256];
     const int t = threadIdx.x;
+    if (t == 0) queue[0] = 0;
     if (t < 256) count[t] = 0;
     __syncthreads();
     auto bucket = [](uint64_t k) {
@@ -1675,6 +1667,145 @@ __global__ __launch_bounds__(1024) void order_by_entry_kernel(const uint64_t *ke
     }
     __syncthreads();
     for (int i = t; i < nq; i += 1024) order[atomicAdd(&start[bucket(key[i])], 1)] = i;
+}
+
+// Upper-layer descent of a whole batch at once (phase 1 of the two-phase schedule) for up to
+// 128 one-byte dimensions (rows of 128 bytes) and 16 links per upper row: 16 lanes per query,
+// lane j owning link j of the current row, so one greedy hop is one links load, one visited
+// probe and one 128-byte vector row per lane and a 16-lane minimum. Same rule and counters as
+// greedy_layer (numba_engine.py:98-148 with ef = 1): a hop moves to the first new neighbour
+// holding the row's minimum when it is strictly closer, and the layer ends after the first hop
+// that does not improve. The visited set of each query's current layer is a 256-slot
+// open-addressing table of whole ids (id + 1) in LDS; a layer that would hold more than `cap`
+// entries stops the query's descent and flags it (pre_hops = -1, pre_key = all ones, so it
+// sorts first), and the layer-0 launch then runs that query's whole descent itself.
+
+__global__ __launch_bounds__(256) void descend_kernel(SearchArgs a) {
+    __shared__ __attribute__((aligned(16))) uint32_t table[kDescentGroups][kDescentSlots];
+    __shared__ __attribute__((aligned(16))) uint8_t qbytes[kDescentGroups][128];
+    const int lane = lane_id();
+    const int j = lane & 15;
+    const int g = threadIdx.x >> 4;
+    const int first = lane & 48;  // this query's first lane in the wave
+    const int qi = blockIdx.x * kDescentGroups + g;
+    const bool act = qi < a.nq;
+    const int qc = act ? qi : a.nq - 1;  // groups past the batch compute on a valid query
+    uint32_t *const tab = table[g];
+
+    // Query bytes (lane j: bytes 8j..8j+7) and |q|^2, every lane of the block active.
+    int32_t qq = 0;
+    uint32_t qw[2] = {0u, 0u};
+    {
+        const float *q = a.q_f32 + (size_t)qc * a.dim;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int i = 8 * j + t;
+            const int v = i < a.dim ? (int)q[i] : 0;
+            qw[t >> 2] |= (uint32_t)v << (8 * (t & 3));
+            qq += v * v;
+        }
+        *(uint2 *)&qbytes[g][8 * j] = make_uint2(qw[0], qw[1]);
+        qq = group_sum(qq, 16);
+    }
+    auto key_of = [&](int32_t s) -> uint32_t {
+        return (uint32_t)(a.metric == kL2 ? key_of_exact(s + qq, kL2) : key_of_exact(-s, a.metric));
+    };
+    // Entry distance (counted, numba_engine.py:449-451): 8 bytes per lane, summed over the 16.
+    const uint32_t entry = (uint32_t)a.entry;
+    uint32_t cur_k;
+    {
+        const uint2 x = *(const uint2 *)(a.vec_u8 + (size_t)entry * 128 + 8 * j);
+        uint32_t xq = __builtin_amdgcn_udot4(x.x, qw[0], 0u, false);
+        xq = __builtin_amdgcn_udot4(x.y, qw[1], xq, false);
+        uint32_t xx = __builtin_amdgcn_udot4(x.x, x.x, 0u, false);
+        xx = __builtin_amdgcn_udot4(x.y, x.y, xx, false);
+        const int32_t part = a.metric == kL2 ? (int32_t)xx - 2 * (int32_t)xq : (int32_t)xq;
+        cur_k = key_of(group_sum(part, 16));
+    }
+    wave_sync();
+    uint32_t cur_id = entry;
+    int32_t uoff = a.max_level > 0 ? a.upper_off[entry] : 0;
+    int dcount = 1, hops = 0;
+    bool flagged = false;
+    if (act) {
+        for (int layer = a.max_level; layer > 0 && !flagged; --layer) {
+#pragma unroll
+            for (int s = 0; s < kDescentSlots / 64; ++s)
+                *(uint4 *)&tab[64 * s + 4 * j] = make_uint4(0u, 0u, 0u, 0u);
+            wave_sync();
+            if (j == 0) tab[(cur_id * kHashMul) >> 24] = cur_id + 1u;
+            wave_sync();
+            int count = 1;
+            for (;;) {
+                if (count + 16 > a.descent_cap) {
+                    flagged = true;
+                    break;
+                }
+                hops += 1;
+                const int32_t nb = a.links_up[((int64_t)uoff + layer - 1) * 16 + j];
+                bool isnew = false;
+                if (nb >= 0) {  // the ids of a row are distinct
+                    const uint32_t v = (uint32_t)nb + 1u;
+                    uint32_t pos = ((uint32_t)nb * kHashMul) >> 24;
+                    for (;;) {
+                        const uint32_t old = atomicCAS(&tab[pos], 0u, v);
+                        if (old == 0u) {
+                            isnew = true;
+                            break;
+                        }
+                        if (old == v) break;
+                        pos = (pos + 1u) & (kDescentSlots - 1);
+                    }
+                }
+                const uint32_t gm = (uint32_t)(ballot(isnew) >> first) & 0xFFFFu;
+                count += __popc(gm);
+                dcount += __popc(gm);
+                uint32_t k32 = ~0u;
+                int32_t nuo = 0;
+                if (isnew) {
+                    const uint4 *row = (const uint4 *)(a.vec_u8 + (size_t)nb * 128);
+                    const uint4 *qv = (const uint4 *)qbytes[g];
+                    uint32_t xq = 0u, xx = 0u;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        const uint4 x = row[c];
+                        const uint4 y = qv[c];
+                        xq = __builtin_amdgcn_udot4(x.x, y.x, xq, false);
+                        xq = __builtin_amdgcn_udot4(x.y, y.y, xq, false);
+                        xq = __builtin_amdgcn_udot4(x.z, y.z, xq, false);
+                        xq = __builtin_amdgcn_udot4(x.w, y.w, xq, false);
+                        xx = __builtin_amdgcn_udot4(x.x, x.x, xx, false);
+                        xx = __builtin_amdgcn_udot4(x.y, x.y, xx, false);
+                        xx = __builtin_amdgcn_udot4(x.z, x.z, xx, false);
+                        xx = __builtin_amdgcn_udot4(x.w, x.w, xx, false);
+                    }
+                    nuo = a.upper_off[nb];
+                    k32 = key_of(a.metric == kL2 ? (int32_t)xx - 2 * (int32_t)xq : (int32_t)xq);
+                }
+                // Minimum over the query's 16 lanes (one DPP row).
+                uint32_t m = k32;
+                m = min(m, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)m, 0xB1, 0xF, 0xF, true));
+                m = min(m, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)m, 0x4E, 0xF, 0xF, true));
+                m = min(m, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)m, 0x141, 0xF, 0xF, true));
+                m = min(m, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)m, 0x140, 0xF, 0xF, true));
+                const uint32_t hit = (uint32_t)(ballot(isnew && k32 == m) >> first) & 0xFFFFu;
+                const int src = first + (hit ? __ffs(hit) - 1 : 0);
+                const uint32_t nid = (uint32_t)shfl(nb, src);
+                const int32_t noff = shfl(nuo, src);
+                if (!(m < cur_k)) break;
+                cur_k = m;
+                cur_id = nid;
+                uoff = noff;
+            }
+            wave_sync();
+        }
+    }
+    if (act && j == 0) {
+        a.pre_key[qi] = flagged ? kInfKey : (uint64_t)cur_k;
+        a.pre_id[qi] = cur_id;
+        a.pre_dc[qi] = dcount;
+        a.pre_hops[qi] = flagged ? -1 : hops;
+    }
 }
 
 template __global__ void search_kernel<kExactU8, false>(SearchArgs);
@@ -1758,17 +1889,16 @@ hipError_t search_launch(int mode, bool filt, const SearchArgs &a, int blocks, h
         hipLaunchKernelGGL(fn, dim3(blocks), dim3(64), a.lds_per_wave, s, a);
         return hipGetLastError();
     }
-    // Two phases: every descent, the order, then layer 0 longest first.
-    SearchArgs d = a;
-    d.phase = 1;
-    d.order = nullptr;
-    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, s, a.queue);
-    hipLaunchKernelGGL(fn, dim3(blocks), dim3(64), a.lds_per_wave, s, d);
+    // Two phases: every descent (16 lanes per query), the order, then layer 0 longest first.
+    if (a.nq < 1 || a.w1 != 16 || a.dimp != 128 || a.dim > 128 || mode != kExactU8 ||
+        a.descent_cap < 1 || a.descent_cap > kDescentMaxCap)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(descend_kernel, dim3((a.nq + kDescentGroups - 1) / kDescentGroups),
+                       dim3(256), 0, s, a);
     hipLaunchKernelGGL(order_by_entry_kernel, dim3(1), dim3(1024), 0, s, a.pre_key, a.nq,
-                       const_cast<int32_t *>(a.order));
+                       const_cast<int32_t *>(a.order), a.queue);
     SearchArgs l = a;
     l.phase = 2;
-    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, s, a.queue);
     hipLaunchKernelGGL(fn, dim3(blocks), dim3(64), a.lds_per_wave, s, l);
     return hipGetLastError();
 }

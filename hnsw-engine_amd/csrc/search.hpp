@@ -48,14 +48,16 @@ struct SearchArgs {
     int32_t *out_dc;
     int32_t *out_hops;
     int32_t *out_fail;  // 1: candidate spill overflowed, rerun with a larger spill
-    // Two-phase schedule of the register beam: phase 1 runs each query's upper-layer descent
-    // and stores its layer-0 entry (key, id) and counters; phase 2 starts layer 0 from them,
-    // taking queries in `order` (longest expected first). phase 0: one launch does both.
+    // Two-phase schedule of the register beam: descend_kernel runs every query's upper-layer
+    // descent and stores its layer-0 entry (key, id) and counters (pre_hops = -1: not done,
+    // the layer-0 launch descends itself); phase 2 starts layer 0 from them, taking queries in
+    // `order` (longest expected first). phase 0: one launch does both.
     int phase;
     const int32_t *order;  // queue position -> query index (null: identity)
     uint64_t *pre_key;
     uint32_t *pre_id;
     int32_t *pre_dc, *pre_hops;
+    int descent_cap;  // descend_kernel: visited entries per layer before a query falls back
     // persistent work queue and per-wave scratch (slot = global wave index)
     int32_t *queue;
     uint32_t *bitsets;
@@ -88,6 +90,12 @@ struct SearchArgs {
     // hash to the global bitset, [1] queries whose evicted ties outgrew the register beam
     int32_t *stats;
 };
+
+// descend_kernel (phase 1 of the two-phase schedule): queries per block of 256 threads, visited
+// slots per query, and the largest fill of those slots it allows (3/4, so every probe ends).
+constexpr int kDescentGroups = 16;
+constexpr int kDescentSlots = 256;
+constexpr int kDescentMaxCap = 192;
 
 // Learned-stop model parameters (float32, weights already rounded to bfloat16):
 // [0,16) feature means, [16,32) inverse standard deviations, [32, 32+16*64) W1 (row i: feature

@@ -128,7 +128,7 @@ def lib() -> ctypes.CDLL:
             "hm_mlp_forward": ([vp, i64, vp, vp], ctypes.c_int),
             "hm_mlp_train": ([vp, vp, i64, vp, i32, i32, ctypes.c_float, ctypes.c_float,
                               ctypes.c_uint64, vp], ctypes.c_int),
-            "hm_set_tuning": ([vp, i32, i32, i32, i32, i32], ctypes.c_int),
+            "hm_set_tuning": ([vp, i32, i32, i32, i32, i32, i32], ctypes.c_int),
             "hm_read_stats": ([vp, vp, ctypes.c_int], ctypes.c_int),
             "hm_last_error": ([], ctypes.c_char_p),
         }

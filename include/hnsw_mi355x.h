@@ -206,11 +206,14 @@ int hm_mlp_train(const float *x, const float *y, int64_t n, float *params, int32
  * register-beam kernel and of the general kernel (larger hashes spill to the global bitset
  * later but leave fewer waves per compute unit), a cap on resident waves per compute unit, a
  * cap on the probe distance of the register-beam kernel's 16-bit visited slots (beyond it the
- * visited set moves to the global bitset), and one_launch = 1 to run a register-beam batch
- * with more queries than wave slots in one launch instead of descents first and layer 0
- * longest-first. None of them changes results. */
+ * visited set moves to the global bitset), one_launch = 1 to run a register-beam batch with
+ * more queries than wave slots in one launch instead of all descents first (16 lanes per query)
+ * and layer 0 longest-first, and descent_cap (1..192): the visited entries one layer of that
+ * batched descent may hold before the query's descent moves into the layer-0 launch. None of
+ * them changes results. */
 int hm_set_tuning(hm_index *index, int32_t fast_hash_log2, int32_t hash_log2,
-                  int32_t waves_per_cu, int32_t probe_limit, int32_t one_launch);
+                  int32_t waves_per_cu, int32_t probe_limit, int32_t one_launch,
+                  int32_t descent_cap);
 
 /* Event counters since the last reset: out[0] layer searches whose visited set outgrew the LDS
  * hash, out[1] queries whose evicted ties outgrew the register beam (re-run on the general

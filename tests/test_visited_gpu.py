@@ -25,9 +25,10 @@ def _same(got, want, msg=""):
         np.testing.assert_array_equal(g, w, err_msg=f"{msg} {name}")
 
 
-def _tune(eng, fast_hash_log2=0, probe_limit=0, waves_per_cu=0, one_launch=0) -> None:
+def _tune(eng, fast_hash_log2=0, probe_limit=0, waves_per_cu=0, one_launch=0,
+          descent_cap=0) -> None:
     _native.check(_native.lib().hm_set_tuning(eng._device(), fast_hash_log2, 0, waves_per_cu,
-                                              probe_limit, one_launch))
+                                              probe_limit, one_launch, descent_cap))
 
 
 def _moves(eng) -> int:
@@ -37,11 +38,11 @@ def _moves(eng) -> int:
     return int(out[0])
 
 
-def _build(n, dim, M, seed):
+def _build(n, dim, M, seed, metric="l2"):
     data, queries = clustered_u8(n, dim, 400, n_clusters=60, sigma=30.0, seed=seed)
     levels = np.floor(-np.log(1.0 - np.random.default_rng(seed + 1).random(n))
                       / np.log(M)).astype(np.int64)
-    eng = MI355XEngine(dim, "l2", M, 100, "heuristic", False, False)
+    eng = MI355XEngine(dim, metric, M, 100, "heuristic", False, False)
     eng.insert(data, levels)
     return eng, data, queries
 
@@ -53,13 +54,19 @@ def m16d128():
 
 
 @pytest.fixture(scope="module")
+def m16d128_ip():
+    """6,000 x 128, M = 16, inner product: the constant shape with the other exact metric."""
+    return _build(6_000, 128, 16, 37, "ip")
+
+
+@pytest.fixture(scope="module")
 def wide_ids():
     """70,000 x 16, M = 8: 17-bit ids, so a 64-slot table needs 32-bit slots."""
     return _build(70_000, 16, 8, 41)
 
 
-def _oracle(eng, queries, k, ef):
-    return oracle.search_batch(eng.native_arrays(), eng.entry_point, eng.max_level, "l2",
+def _oracle(eng, queries, k, ef, metric="l2"):
+    return oracle.search_batch(eng.native_arrays(), eng.entry_point, eng.max_level, metric,
                                queries, k, ef, threads=8)
 
 
@@ -121,21 +128,40 @@ def test_four_entry_beam_mixed_batch_matches_oracle(m16d128) -> None:
 
 @pytest.mark.parametrize("ef,k", [(48, 10), (96, 100), (300, 10)])
 def test_two_phase_schedule_matches_oracle(m16d128, ef: int, k: int) -> None:
-    """With fewer wave slots than queries (one wave per CU) a batch on the multi-entry register
-    beams (ef > 64) runs its descents first and layer 0 longest-first; one launch gives the same
-    results (ef = 48 runs in one launch either way)."""
+    """With fewer wave slots than queries (one wave per CU) a batch on the register beams runs
+    every descent first (16 lanes per query) and layer 0 longest-first. One launch gives the
+    same results, and so do descent caps that move some (20) or all (1) of the descents into the
+    layer-0 launch. 397 queries: the last block of the descent kernel holds 13 of its 16
+    queries."""
     eng, _, queries = m16d128
+    queries = queries[:397]
     want = _oracle(eng, queries, k, ef)
-    print(f"case m16d128 two-phase ef={ef} k={k}", flush=True)
+    runs = {}
     try:
-        _tune(eng, waves_per_cu=1)
-        two = eng.search_many(queries, k, ef)
-        _tune(eng, waves_per_cu=1, one_launch=1)
-        one = eng.search_many(queries, k, ef)
+        for name, kw in (("two", {}), ("cap20", {"descent_cap": 20}),
+                         ("cap1", {"descent_cap": 1}), ("one", {"one_launch": 1})):
+            print(f"case m16d128 two-phase ef={ef} k={k} {name}", flush=True)
+            _tune(eng, waves_per_cu=1, **kw)
+            runs[name] = eng.search_many(queries, k, ef)
     finally:
         _tune(eng)
-    _same(two, want, f"two-phase ef={ef}")
-    _same(one, want, f"one launch ef={ef}")
+    for name, got in runs.items():
+        _same(got, want, f"{name} ef={ef}")
+
+
+@pytest.mark.parametrize("ef", [32, 100])
+def test_two_phase_inner_product_matches_oracle(m16d128_ip, ef: int) -> None:
+    eng, _, queries = m16d128_ip
+    queries = queries[:397]
+    want = _oracle(eng, queries, 10, ef, "ip")
+    print(f"case m16d128 ip ef={ef}", flush=True)
+    try:
+        _tune(eng, waves_per_cu=1)
+        two = eng.search_many(queries, 10, ef)
+    finally:
+        _tune(eng)
+    _same(two, want, f"ip two-phase ef={ef}")
+    _same(eng.search_many(queries, 10, ef), want, f"ip ef={ef}")
 
 
 def test_wide_beam_on_the_bitset_matches_oracle(m16d128) -> None:
@@ -168,6 +194,7 @@ def test_whole_id_slots_match_oracle(wide_ids) -> None:
 
 
 def test_constant_shape_stop_kernel_that_never_fires_is_the_plain_search(m16d128) -> None:
+    """In one launch and (one wave per CU) after the batched descent."""
     eng, _, queries = m16d128
     rng = np.random.default_rng(3)
     x = rng.standard_normal((2000, learned.N_FEATURES)).astype(np.float32)
@@ -176,6 +203,11 @@ def test_constant_shape_stop_kernel_that_never_fires_is_the_plain_search(m16d128
     eng.set_stop_model(p, 0.5)
     try:
         got = eng.search_many(queries, 10, 48, learned=True)
+        _tune(eng, waves_per_cu=1)
+        two = eng.search_many(queries, 10, 48, learned=True)
     finally:
+        _tune(eng)
         eng.set_stop_model(None)
-    _same(got, _oracle(eng, queries, 10, 48), "stop kernel")
+    want = _oracle(eng, queries, 10, 48)
+    _same(got, want, "stop kernel")
+    _same(two, want, "stop kernel after the batched descent")
