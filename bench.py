@@ -14,7 +14,8 @@ fastest setting with recall@10 >= 0.95. Then W untimed and K timed steps of the 
 10,000 queries with queries and outputs already in HBM. The JSON line carries the roofline of
 one batch search (its kernels: the batched upper-layer descent, the longest-first order, the
 layer-0 beam; algorithmic bytes per batch over the batch's average duration, HIP events on the
-launch stream) and the CPU baseline (the C oracle at the plain beam's ef, 1 thread and all
+launch stream), the same batches replayed from a captured HIP graph (``config.graph_replay``)
+and the CPU baseline (the C oracle at the plain beam's ef, 1 thread and all
 cores, on a bounded query sample, ids checked equal to the GPU's plain results).
 """
 
@@ -227,6 +228,32 @@ def main() -> int:
     ids_final = ids_dev.cpu().numpy()
     assert np.array_equal(ids_final, chosen["ids"]), "non-deterministic search results"
 
+    # The same batches replayed from a captured HIP graph (hm_search_graph_dev, on a stream of
+    # its own), reported beside the direct launches.
+    gstream = torch.cuda.Stream()
+    with torch.cuda.stream(gstream):
+        opts = _native.SearchOpts()
+        opts.nq, opts.k_all, opts.ef_all, opts.kstride, opts.mask_all = nq, k, ef, k, -1
+        opts.learned = int(use_learned)
+        gargs = (handle, ctypes.c_void_p(q_dev.data_ptr()), ctypes.byref(opts), None,
+                 int(exact_u8), ctypes.c_void_p(ids_dev.data_ptr()),
+                 ctypes.c_void_p(d_dev.data_ptr()), ctypes.c_void_p(dc_dev.data_ptr()),
+                 ctypes.c_void_p(hops_dev.data_ptr()), ctypes.c_void_p(fail_dev.data_ptr()),
+                 ctypes.c_void_p(gstream.cuda_stream))
+        for _ in range(max(2, args.warmup)):  # the first call captures
+            _native.check(lib.hm_search_graph_dev(*gargs))
+        gstream.synchronize()
+        g0 = torch.cuda.Event(enable_timing=True)
+        g1 = torch.cuda.Event(enable_timing=True)
+        g0.record(gstream)
+        for _ in range(args.steps):
+            _native.check(lib.hm_search_graph_dev(*gargs))
+        g1.record(gstream)
+        gstream.synchronize()
+    graph_ms = g0.elapsed_time(g1) / args.steps
+    assert np.array_equal(ids_dev.cpu().numpy(), chosen["ids"]), "graph replay differs"
+    graph_info = {"ms_per_step": graph_ms, "qps": nq / (graph_ms / 1e3)}
+
     # Algorithmic bytes of one launch: every distance reads one stored row, every expansion one
     # links row (upper-layer rows counted at the layer-0 width), as the reference counts them.
     row_bytes = eng.dim if exact_u8 else eng.dim * 4
@@ -273,6 +300,7 @@ def main() -> int:
             "sweep": [{kk: s[kk] for kk in ("ef", "recall", "qps", "dcount_mean", "hops_mean")}
                       for s in sweep],
             "learned": learned_info,
+            "graph_replay": graph_info,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

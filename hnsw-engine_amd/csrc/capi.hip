@@ -481,18 +481,13 @@ int hm_search(hm_index *h, const float *queries, const hm_search_opts *o, int64_
                        hops);
 }
 
-int hm_search_dev(hm_index *h, const float *d_queries, const hm_search_opts *o,
-                  const uint32_t *d_mask_bits, int exact_u8, int64_t *d_ids, double *d_dists,
-                  int32_t *d_dcount, int32_t *d_hops, int32_t *d_fail, void *stream) {
-    if (!h || !d_queries || !o || !d_ids || !d_dists) return invalid("null argument");
-    int st = check_opts(h, o);
-    if (st) return st;
-    if (exact_u8 && !h->exact_u8) return invalid("the index keeps no exact 8-bit copy");
-    if (o->k || o->ef) return invalid("hm_search_dev takes uniform k and ef");
-    if (o->nq == 0) return HM_OK;
-    HM_TRY(hipSetDevice(h->device), "hipSetDevice");
-    hipStream_t s = (hipStream_t)stream;
-    Batch b;
+namespace {
+
+// The batch of hm_search_dev / hm_search_graph_dev (arguments already checked).
+int device_batch(hm_index *h, const float *d_queries, const hm_search_opts *o,
+                 const uint32_t *d_mask_bits, int64_t *d_ids, double *d_dists, int32_t *d_dcount,
+                 int32_t *d_hops, int32_t *d_fail, Batch &b, bool &filt) {
+    b = Batch();
     b.nq = o->nq;
     b.q32 = d_queries;
     b.kstride = o->kstride;
@@ -503,7 +498,7 @@ int hm_search_dev(hm_index *h, const float *d_queries, const hm_search_opts *o,
     b.mask_words = (h->n + 31) / 32;
     b.midx = o->mask_index;
     b.mask_all = o->mask_all;
-    const bool filt = d_mask_bits != nullptr && (o->mask_index || o->mask_all >= 0);
+    filt = d_mask_bits != nullptr && (o->mask_index || o->mask_all >= 0);
     HM_TRY(h->dc.ensure(o->nq), "alloc");
     HM_TRY(h->hops.ensure(o->nq), "alloc");
     HM_TRY(h->fail.ensure(o->nq), "alloc");
@@ -513,8 +508,104 @@ int hm_search_dev(hm_index *h, const float *d_queries, const hm_search_opts *o,
     b.hops = d_hops ? d_hops : h->hops.p;
     b.fail = d_fail ? d_fail : h->fail.p;
     b.learned = o->learned != 0;
+    return HM_OK;
+}
+
+int check_dev(hm_index *h, const float *d_queries, const hm_search_opts *o, int exact_u8,
+              int64_t *d_ids, double *d_dists) {
+    if (!h || !d_queries || !o || !d_ids || !d_dists) return invalid("null argument");
+    int st = check_opts(h, o);
+    if (st) return st;
+    if (exact_u8 && !h->exact_u8) return invalid("the index keeps no exact 8-bit copy");
+    if (o->k || o->ef) return invalid("device batches take uniform k and ef");
+    return HM_OK;
+}
+
+// Everything a captured batch bakes in: its arguments, the tuning and stop model, and the
+// address and size of every scratch buffer its kernels use.
+std::vector<int64_t> graph_key(const hm_index *h, const float *d_queries, const hm_search_opts *o,
+                               const uint32_t *d_mask_bits, int exact_u8, const Batch &b) {
+    auto P = [](const void *p) { return (int64_t)(intptr_t)p; };
+    float logit = h->stop_logit;
+    int32_t logit_bits;
+    memcpy(&logit_bits, &logit, 4);
+    return {P(d_queries), o->nq, o->k_all, o->ef_all, o->kstride, o->mask_all, P(o->mask_index),
+            o->learned, P(d_mask_bits), exact_u8, P(b.ids), P(b.dists), P(b.dc), P(b.hops),
+            P(b.fail), h->tune_fast_hlog, h->tune_hlog, h->tune_waves_per_cu, h->tune_probe_limit,
+            h->tune_schedule, h->tune_descent_cap, h->stop_on, logit_bits, h->stop_tmin,
+            P(h->stop_w.p), P(h->bits.p), (int64_t)h->bits.cap, P(h->vlist.p),
+            (int64_t)h->vlist.cap, P(h->spill_key.p), (int64_t)h->spill_key.cap,
+            P(h->spill_id.p), (int64_t)h->spill_id.cap, P(h->queue.p), P(h->stats.p),
+            P(h->pre_key.p), (int64_t)h->pre_key.cap, P(h->pre_id.p), (int64_t)h->pre_id.cap,
+            P(h->pre_dc.p), (int64_t)h->pre_dc.cap, P(h->pre_hops.p), (int64_t)h->pre_hops.cap,
+            P(h->order.p), (int64_t)h->order.cap};
+}
+
+}  // namespace
+
+int hm_search_dev(hm_index *h, const float *d_queries, const hm_search_opts *o,
+                  const uint32_t *d_mask_bits, int exact_u8, int64_t *d_ids, double *d_dists,
+                  int32_t *d_dcount, int32_t *d_hops, int32_t *d_fail, void *stream) {
+    int st = check_dev(h, d_queries, o, exact_u8, d_ids, d_dists);
+    if (st || o->nq == 0) return st;
+    HM_TRY(hipSetDevice(h->device), "hipSetDevice");
+    Batch b;
+    bool filt = false;
+    st = device_batch(h, d_queries, o, d_mask_bits, d_ids, d_dists, d_dcount, d_hops, d_fail, b,
+                      filt);
+    if (st) return st;
     return launch(h, exact_u8 ? kExactU8 : kSeqF32, filt, b, std::min<int64_t>(h->n, kDefaultSpill),
-                  0, s);
+                  0, (hipStream_t)stream);
+}
+
+int hm_search_graph_dev(hm_index *h, const float *d_queries, const hm_search_opts *o,
+                        const uint32_t *d_mask_bits, int exact_u8, int64_t *d_ids,
+                        double *d_dists, int32_t *d_dcount, int32_t *d_hops, int32_t *d_fail,
+                        void *stream) {
+    int st = check_dev(h, d_queries, o, exact_u8, d_ids, d_dists);
+    if (st || o->nq == 0) return st;
+    if (!stream) return invalid("graph replay needs a stream other than the null stream");
+    HM_TRY(hipSetDevice(h->device), "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    Batch b;
+    bool filt = false;
+    st = device_batch(h, d_queries, o, d_mask_bits, d_ids, d_dists, d_dcount, d_hops, d_fail, b,
+                      filt);
+    if (st) return st;
+    if (h->graph_exec && graph_key(h, d_queries, o, d_mask_bits, exact_u8, b) == h->graph_key) {
+        HM_TRY(hipGraphLaunch(h->graph_exec, s), "graph launch");
+        return HM_OK;
+    }
+    // A new shape: run it directly (this allocates whatever scratch it lacks), then capture the
+    // same launches, which now allocate and clear nothing, for the next call.
+    const int mode = exact_u8 ? kExactU8 : kSeqF32;
+    const int64_t spill = std::min<int64_t>(h->n, kDefaultSpill);
+    st = launch(h, mode, filt, b, spill, 0, s);
+    if (st) return st;
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    h->graph_exec = nullptr;
+    h->graph_key.clear();
+    std::vector<int64_t> key = graph_key(h, d_queries, o, d_mask_bits, exact_u8, b);
+    HM_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "graph capture");
+    st = launch(h, mode, filt, b, spill, 0, s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return fail_hip(e, "graph capture");
+    if (graph_key(h, d_queries, o, d_mask_bits, exact_u8, b) != key) {  // cannot happen
+        (void)hipGraphDestroy(g);
+        return invalid("scratch changed during graph capture");
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return fail_hip(ei, "graph instantiate");
+    h->graph_exec = x;
+    h->graph_key = key;
+    return HM_OK;
 }
 
 int hm_trace_dev(hm_index *h, const float *d_queries, int64_t nq, int32_t k, int32_t ef,

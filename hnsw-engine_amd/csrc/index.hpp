@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "common.hpp"
 
 struct hm_index {
@@ -44,6 +46,9 @@ struct hm_index {
     int tune_schedule = 0;  // 1: one launch per batch (no descents-first schedule)
     int tune_descent_cap = 0;  // > 0: descend_kernel visited entries per layer (else the maximum)
     hm::DevBuf<int32_t> stats;  // event counters, see SearchArgs::stats
+    // hm_search_graph_dev: the captured launches of the last batch shape and what they bake in
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<int64_t> graph_key;
 
     void release_all() {
         hm::DevBuf<int32_t> *i32[] = {&links0, &links_up, &upper, &queue,    &kq,   &efq,
@@ -67,6 +72,9 @@ struct hm_index {
         dists.release();
         stop_w.release();
         stats.release();
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        graph_exec = nullptr;
+        graph_key.clear();
     }
 };
 

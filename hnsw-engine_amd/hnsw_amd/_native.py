@@ -20,6 +20,7 @@ SYMBOLS = (
     "hm_has_exact_u8",
     "hm_search",
     "hm_search_dev",
+    "hm_search_graph_dev",
     "hm_queries_exact",
     "hm_attach_codes",
     "hm_search_codes",
@@ -104,6 +105,10 @@ def lib() -> ctypes.CDLL:
             "hm_has_exact_u8": ([vp], ctypes.c_int),
             "hm_search": ([vp, vp, ctypes.POINTER(SearchOpts), vp, vp, vp, vp], ctypes.c_int),
             "hm_search_dev": (
+                [vp, vp, ctypes.POINTER(SearchOpts), vp, ctypes.c_int, vp, vp, vp, vp, vp, vp],
+                ctypes.c_int,
+            ),
+            "hm_search_graph_dev": (
                 [vp, vp, ctypes.POINTER(SearchOpts), vp, ctypes.c_int, vp, vp, vp, vp, vp, vp],
                 ctypes.c_int,
             ),

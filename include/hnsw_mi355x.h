@@ -113,6 +113,17 @@ int hm_search_dev(hm_index *index, const float *d_queries, const hm_search_opts 
                   const uint32_t *d_mask_bits, int exact_u8, int64_t *d_ids, double *d_dists,
                   int32_t *d_dcount, int32_t *d_hops, int32_t *d_fail, void *stream);
 
+/* hm_search_dev replayed from a HIP graph: the first call with a given set of arguments (and
+ * tuning, stop model and scratch buffers) runs the batch directly and captures its launches;
+ * later calls with the same arguments launch the captured graph on `stream` (which must not be
+ * the null stream), skipping the host-side set-up and per-kernel launch costs. No counterpart in
+ * the reference (its engine has no launches); for serving loops that reuse fixed buffers. Any
+ * change of argument, tuning or scratch recaptures. Results equal hm_search_dev's. */
+int hm_search_graph_dev(hm_index *index, const float *d_queries, const hm_search_opts *opts,
+                        const uint32_t *d_mask_bits, int exact_u8, int64_t *d_ids,
+                        double *d_dists, int32_t *d_dcount, int32_t *d_hops, int32_t *d_fail,
+                        void *stream);
+
 /* 1 if every value of the host batch is an integer in 0..255 (exact 8-bit kernel eligible). */
 int hm_queries_exact(const float *queries, int64_t count);
 
