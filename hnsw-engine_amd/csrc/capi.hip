@@ -70,15 +70,13 @@ void plan(SearchArgs &a, int mode, const hm_index *h, int max_ef, bool fast, int
     a.fast = fast ? 1 : 0;
     a.nr = nr;
     if (fast) {
-        // 2048 slots (4 KB of 16-bit slots for up to 2M ids): measured on the 1M x 128
-        // workload for ef 32..64 it beats 4096 slots (fewer waves fit) although ~6 % of queries
-        // at ef 48 outgrow it and continue on the global bitset; 1024 slots overflow for nearly
-        // every query. The two-slot beam (ef 65..128) too: at ef 96 and 128 4096 slots avoid
-        // nearly all moves to the bitset but leave 15 waves per CU instead of 23, and measured
-        // 2 % slower. The four- and eight-slot beams (ef 129..512, 1,250-1,400 visits) take
-        // 4096 slots.
-        a.h_log2 = h->tune_fast_hlog ? h->tune_fast_hlog
-                                     : (nr >= 4 ? 12 : (max_ef <= 8 ? 10 : 11));
+        // 4096 slots (8 KB of 16-bit slots for up to 4M ids). Measured on 10,000 queries over
+        // 1M x 128 with the longest-first schedule, against 2048 slots (which fit more waves per
+        // CU but send 2-10 % of the layer searches to the global bitset): ef 16 / 32 / 48 / 64 /
+        // 96 / 128 -9 / -9 / -8 / -17 / -3 / -5 %, and -4 % on 2,500 queries in one launch;
+        // 8192 slots are 30-40 % slower (too few waves). The four- and eight-slot beams
+        // (ef 129..512, 1,250-1,400 visits) need 4096 slots all the more.
+        a.h_log2 = h->tune_fast_hlog ? h->tune_fast_hlog : (max_ef <= 8 ? 10 : 12);
         a.r_cap = 64 * nr;
         a.c_cap = 64;
         a.lds_per_wave = lds_bytes_fast(mode, h->dim, h->dimp, a.h_log2, a.id_bits, nr);
@@ -134,7 +132,7 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, 
     const int wide_nr = b.max_ef <= 128 && b.kstride <= 128   ? 2
                         : b.max_ef <= 256 && b.kstride <= 256 ? 4
                         : b.max_ef <= 512 && b.kstride <= 512 ? 8 : 0;
-    const int wide_hlog = h->tune_fast_hlog ? h->tune_fast_hlog : (wide_nr >= 4 ? 12 : 11);
+    const int wide_hlog = h->tune_fast_hlog ? h->tune_fast_hlog : 12;
     const bool wide = allow_fast && !filt && !fast && !b.trace && mode == kExactU8 &&
                       wide_nr > 0 && h->w0 == 32 && h->w1 == 16 && h->dimp == 128 &&
                       a.id_bits - wide_hlog <= 10;

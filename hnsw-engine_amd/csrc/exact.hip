@@ -22,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -514,11 +515,15 @@ struct TilePlan {
     int64_t chunk = 0;
 };
 
+// Tile-shape overrides of the range search (hm_set_exact_tuning; 0: automatic), process-wide.
+std::atomic<int> g_tile_qb{0}, g_tile_sample{0};
+
 TilePlan plan_tile(int64_t m, int64_t nqb, int nch, int k) {
     TilePlan p;
     p.nch = nch;
     const int qbmax = nch <= 4 ? 4 : 2;
     p.qb = nqb <= 256 ? 1 : (nqb <= 512 || qbmax == 2 ? 2 : 4);
+    if (g_tile_qb.load() > 0) p.qb = g_tile_qb.load();
     p.qb = std::min(p.qb, qbmax);
     p.groups = (int)((nqb + kTileWaves * p.qb * 32 - 1) / (kTileWaves * p.qb * 32));
     // About four work-groups per compute unit of MI355X (256 CUs), chunks of >= 4096 rows.
@@ -531,6 +536,7 @@ TilePlan plan_tile(int64_t m, int64_t nqb, int nch, int k) {
     // sixteenth makes the bound pass 2x cheaper but the looser threshold costs more in the range
     // pass (more 16-row groups reach it) and in the collect.
     p.sample_every = std::max(1, std::min(4, nsb / 2));
+    if (g_tile_sample.load() > 0) p.sample_every = std::min(g_tile_sample.load(), nsb);
     p.T = k <= 32 ? 4 : 8;
     // The threshold admits about k * sample_every rows per query (the k-th of a 1/sample_every
     // sample; twice that for an unlucky sample), spread over 2 * nchunks record lists: room for
@@ -816,6 +822,15 @@ extern "C" int hm_exact_dev(const float *d_data, int64_t n, int32_t dim, int32_t
     const bool int8 = exact_u8 != 0 && metric != HM_COSINE && dim <= 1024;
     return exact_device(d_data, n, dim, metric, d_queries, nq, k, d_allowed_bits, int8, d_ids,
                         d_dists, (hipStream_t)stream, device);
+}
+
+extern "C" int hm_set_exact_tuning(int32_t query_blocks, int32_t sample_every) {
+    if (!(query_blocks == 0 || query_blocks == 1 || query_blocks == 2 || query_blocks == 4) ||
+        sample_every < 0 || sample_every > 64)
+        return invalid("tuning values out of range");
+    g_tile_qb.store(query_blocks);
+    g_tile_sample.store(sample_every);
+    return HM_OK;
 }
 
 extern "C" int hm_exact_timing(double *out) {

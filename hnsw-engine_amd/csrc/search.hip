@@ -1618,19 +1618,19 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
     search_loop<MODE, FILT, false, 0, kShapeAny>(a);
 }
 
-// Register-beam kernel: at least four waves per SIMD (<= 128 VGPRs), six (<= 80) for the
-// constant shape with one or two entries per lane, whose LDS fits 24 or 23 waves per CU (four
-// entries per lane: 12 KB of LDS, 13 waves per CU, so four per SIMD and no spills).
+// Register-beam kernel: four waves per SIMD (<= 128 VGPRs, no spills at one to four entries per
+// lane), which is what the LDS allows anyway with 4096 visited slots (constant shape: 10 KB per
+// wave with one entry per lane, 16 waves per CU).
 template <int MODE, int AUX, int SH>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SH == kShapeM16D128 || SH == kShapeM16D128Wide ? 6 : 4))) void search_fast_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void search_fast_kernel(
     SearchArgs a) {
     search_loop<MODE, false, true, AUX, SH>(a);
 }
 
-// Register-beam kernel with the learned stop (its model in LDS): five waves per SIMD
-// (<= 96 VGPRs) for the constant shape, four otherwise.
+// Register-beam kernel with the learned stop (its model in LDS): three waves per SIMD
+// (<= 168 VGPRs), the most its LDS allows with 4096 visited slots.
 template <int MODE, int SH>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SH == kShapeM16D128 ? 5 : 4))) void search_stop_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void search_stop_kernel(
     SearchArgs a) {
     search_loop<MODE, false, true, 2, SH>(a);
 }

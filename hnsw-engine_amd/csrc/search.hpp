@@ -120,7 +120,7 @@ __host__ __device__ inline int visited_hash_bytes(int h_log2, int id_bits) {
 // Bytes of LDS one wave of the register-beam kernel needs: query, (64 nr)-entry R scatter buffer
 // with its flags (shared with the new-neighbour staging), 64-entry tie list and the visited
 // hash (layer 0 with 1 << h_log2 slots, upper layers with at most 512). 128 dimensions, 1M ids
-// and 2,048 slots: 6,016 bytes (24 waves per CU fit).
+// and 4,096 slots: 10,112 bytes (16 waves per CU).
 inline int lds_bytes_fast(int mode, int dim, int dimp, int h_log2, int id_bits, int nr = 1) {
     auto up = [](int v) { return (v + 15) & ~15; };
     int q = mode == kExactU8 ? up(dimp) : up(dim * 8);

@@ -71,3 +71,21 @@ def test_exact_query_check_accepts_only_8bit_integers() -> None:
     q3[7, 7] = -0.0
     assert lib.hm_queries_exact(_native.ptr(q3), q3.size) == 1
     assert lib.hm_queries_exact(_native.ptr(q), 0) == 1
+
+
+def test_exact_tuning_checks_its_range() -> None:
+    """Argument checks only (host code): out-of-range tile shapes are refused."""
+    lib = _native.lib()
+    for qb, se in ((3, 0), (8, 0), (0, -1), (0, 65)):
+        with pytest.raises(ValueError, match="out of range"):
+            _native.check(lib.hm_set_exact_tuning(qb, se))
+    _native.check(lib.hm_set_exact_tuning(0, 0))
+
+
+def test_autotune_grids_enumerate_every_setting() -> None:
+    from hnsw_amd import autotune
+
+    s = autotune._settings(autotune.SEARCH_GRID)
+    assert len(s) == 3 * 2 * 2 and len({tuple(sorted(d.items())) for d in s}) == len(s)
+    e = autotune._settings(autotune.EXACT_GRID)
+    assert len(e) == 9 and all(set(d) == {"query_blocks", "sample_every"} for d in e)
