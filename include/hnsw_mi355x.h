@@ -222,8 +222,8 @@ int hm_mlp_train(const float *x, const float *y, int64_t n, float *params, int32
 /* Tuning knobs (0 restores the automatic choice): log2 of the visited-hash slots per wave of the
  * register-beam kernel and of the general kernel (larger hashes spill to the global bitset
  * later but leave fewer waves per compute unit), a cap on resident waves per compute unit, a
- * cap on the probe distance of the register-beam kernel's 16-bit visited slots (beyond it the
- * visited set moves to the global bitset), one_launch = 1 to run a register-beam batch with
+ * cap on the probe distance of the 16-bit visited slots (beyond it the visited set moves to the
+ * global bitset), one_launch = 1 to run a register-beam batch with
  * more queries than wave slots in one launch instead of all descents first (16 lanes per query)
  * and layer 0 longest-first, and descent_cap (1..192): the visited entries one layer of that
  * batched descent may hold before the query's descent moves into the layer-0 launch. None of
