@@ -470,24 +470,30 @@ HM_DEV bool visit_quotient(const Lds &L, Visited &V, uint32_t id, bool active) {
     if (V.probe_limit > 0 && (uint32_t)V.probe_limit < maxd) maxd = (uint32_t)V.probe_limit;
     bool isnew = false, full = false;
     if (active) {
-        uint32_t dist = 0u, w = 0u;  // w: contents (or guess) of the word holding the slot
-        int state;                   // 0 probing, 1 inserted, 2 present, 3 out of distance bits
+        // The probe position advances incrementally: word index, the slot's half of it (shift 0
+        // or 16) and this lane's content at the current distance (the distance field is the low
+        // bits, so the next distance's content is one more).
+        const uint32_t wmask = hmask >> 1;
+        const uint32_t last = (rem << db) | (maxd + 1u);  // content at the largest distance
+        uint32_t word = home >> 1, sh = (home & 1u) << 4, mine = (rem << db) | 1u;
+        uint32_t w = 0u;  // contents (or guess) of the word holding the slot
+        bool go;
         do {  // branch-free step, one LDS operation: claim a slot that looks empty, else read
-            const uint32_t slot = (home + dist) & hmask;
-            const uint32_t sh = (slot & 1u) << 4;
-            const uint32_t mine = (rem << db) | (dist + 1u);
             const uint32_t cur = (w >> sh) & 0xFFFFu;
-            const uint32_t old = atomicCAS(&L.hash[slot >> 1], w, cur == 0u ? w | (mine << sh) : w);
-            const bool claimed = old == w && cur == 0u;
+            const uint32_t old = atomicCAS(&L.hash[word], w, cur == 0u ? w | (mine << sh) : w);
             const uint32_t now = (old >> sh) & 0xFFFFu;
+            const bool claimed = old == w && cur == 0u;
             const bool other = !claimed && now != mine && now != 0u;
-            state = claimed ? 1 : (now == mine ? 2 : (other && dist == maxd ? 3 : 0));
-            const bool step = other && dist != maxd;
-            dist += step ? 1u : 0u;
-            w = step && sh != 0u ? 0u : old;  // the slot after an odd one starts a new word
-        } while (state == 0);
-        isnew = state == 1;
-        full = state == 3;
+            const bool step = other && mine != last;
+            isnew = claimed;
+            full = other && mine == last;
+            go = step || (!claimed && now == 0u);  // next slot, or this one again with its word
+            const bool next_word = step && sh != 0u;  // the slot after an odd one starts a word
+            mine += step ? 1u : 0u;
+            sh = step ? sh ^ 16u : sh;
+            word = (word + (next_word ? 1u : 0u)) & wmask;
+            w = next_word ? 0u : old;
+        } while (go);
     }
     if (ballot(full)) {
         visited_to_bits(L, V);
