@@ -1318,14 +1318,20 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
             for (int s = 0; s < NR; ++s) rkey[s] = key_hi(re[s]);
             const uint64_t all_new = (1ull << nnew) - 1ull;  // nnew <= 32
             const uint32_t worst = readlane_u32(slot_of<NR>(rkey, (ef - 1) >> 6), (ef - 1) & 63);
-            const uint64_t part = rn == ef ? ballot(lane < nnew && nk < worst) : all_new;
+            const uint64_t part = rn == ef ? lanes_lt_u32(nk, worst) & all_new : all_new;
+            uint64_t rvalid[NR];  // lanes of slot s that hold an R entry
+#pragma unroll
+            for (int s = 0; s < NR; ++s) {
+                const int cnt = min(max(rn - 64 * s, 0), 64);
+                rvalid[s] = cnt >= 64 ? ~0ull : ((1ull << cnt) - 1ull);
+            }
             int c_r = 0, c_n = 0;
             for (uint64_t m = part; m; m &= m - 1ull) {
                 const int q = __ffsll((long long)m) - 1;
                 const uint32_t kq = readlane_u32(nk, q);
                 int cnt = 0;
 #pragma unroll
-                for (int s = 0; s < NR; ++s) cnt += popc(ballot(in_r(s, rn) && rkey[s] <= kq));
+                for (int s = 0; s < NR; ++s) cnt += popc(lanes_le_u32(rkey[s], kq) & rvalid[s]);
                 if (lane == q) c_r = cnt;
                 if (q < lane && kq <= nk) c_n += 1;
             }
@@ -1345,7 +1351,7 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
 #pragma unroll
                 for (int s = 0; s < NR; ++s) {
                     if (eq < re[s]) pos_r[s] += 1;
-                    below += popc(ballot(in_r(s, rn) && re[s] < eq));
+                    below += popc(lanes_lt_u64(re[s], eq) & rvalid[s]);
                 }
                 if (lane == q) r_below = below;
                 if (acc && eq < ne) a_below += 1;

@@ -22,6 +22,12 @@ HM_DEV void wave_sync() {
 
 HM_DEV uint64_t ballot(bool p) { return __ballot(p); }
 
+// Masks of the active lanes whose v compares below a wave-uniform s, straight from the compare
+// instruction (a ballot of the same comparison first copies it into a vector register).
+HM_DEV uint64_t lanes_lt_u32(uint32_t v, uint32_t s) { return __builtin_amdgcn_uicmp(v, s, 36); }
+HM_DEV uint64_t lanes_le_u32(uint32_t v, uint32_t s) { return __builtin_amdgcn_uicmp(v, s, 37); }
+HM_DEV uint64_t lanes_lt_u64(uint64_t v, uint64_t s) { return __builtin_amdgcn_uicmpl(v, s, 36); }
+
 HM_DEV int popc(uint64_t m) { return __popcll(m); }
 
 // Number of set bits of `m` below this lane.
