@@ -20,10 +20,12 @@ HM_DEV void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-HM_DEV uint64_t ballot(bool p) { return __ballot(p); }
+// The active lanes whose p holds, as a mask: the condition's own lane mask (HIP's __ballot first
+// copies the condition into a vector register and compares it again).
+HM_DEV uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // Masks of the active lanes whose v compares below a wave-uniform s, straight from the compare
-// instruction (a ballot of the same comparison first copies it into a vector register).
+// instruction.
 HM_DEV uint64_t lanes_lt_u32(uint32_t v, uint32_t s) { return __builtin_amdgcn_uicmp(v, s, 36); }
 HM_DEV uint64_t lanes_le_u32(uint32_t v, uint32_t s) { return __builtin_amdgcn_uicmp(v, s, 37); }
 HM_DEV uint64_t lanes_lt_u64(uint64_t v, uint64_t s) { return __builtin_amdgcn_uicmpl(v, s, 36); }
