@@ -1325,28 +1325,25 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
                 const int cnt = min(max(rn - 64 * s, 0), 64);
                 rvalid[s] = cnt >= 64 ? ~0ull : ((1ull << cnt) - 1ull);
             }
-            int c_r = 0, c_n = 0;
-            for (uint64_t m = part; m; m &= m - 1ull) {
-                const int q = __ffsll((long long)m) - 1;
-                const uint32_t kq = readlane_u32(nk, q);
-                int cnt = 0;
-#pragma unroll
-                for (int s = 0; s < NR; ++s) cnt += popc(lanes_le_u32(rkey[s], kq) & rvalid[s]);
-                if (lane == q) c_r = cnt;
-                if (q < lane && kq <= nk) c_n += 1;
-            }
-            const bool acc = ((part >> lane) & 1ull) && c_r + c_n < ef;
-            const uint64_t am = ballot(acc);
-            if (am == 0) goto expanded;
-            const int na = popc(am);
-            if constexpr (AUX != 0) S.last_acc += na;
+            // One pass over the candidates in link order decides each one (uniformly: #{R keys <=
+            // its key} + #{earlier candidates with keys <= its key} < ef) and, when it is
+            // accepted, adds it to the merge positions: every R entry above it moves up one, it
+            // counts the R entries below it, and every lane counts the accepted entries below its
+            // own entry (only the accepted lanes use that count).
             int pos_r[NR];
 #pragma unroll
             for (int s = 0; s < NR; ++s) pos_r[s] = lane + 64 * s;
             int r_below = 0, a_below = 0;
-            for (uint64_t m = am; m; m &= m - 1ull) {
+            uint64_t am = 0;
+            for (uint64_t m = part; m; m &= m - 1ull) {
                 const int q = __ffsll((long long)m) - 1;
                 const uint64_t eq = readlane_u64(ne, q);
+                const uint32_t kq = (uint32_t)(eq >> 32);
+                int c = popc(lanes_le_u32(nk, kq) & part & ((1ull << q) - 1ull));
+#pragma unroll
+                for (int s = 0; s < NR; ++s) c += popc(lanes_le_u32(rkey[s], kq) & rvalid[s]);
+                if (c >= ef) continue;
+                am |= 1ull << q;
                 int below = 0;
 #pragma unroll
                 for (int s = 0; s < NR; ++s) {
@@ -1354,8 +1351,12 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
                     below += popc(lanes_lt_u64(re[s], eq) & rvalid[s]);
                 }
                 if (lane == q) r_below = below;
-                if (acc && eq < ne) a_below += 1;
+                if (eq < ne) a_below += 1;
             }
+            if (am == 0) goto expanded;
+            const bool acc = (am >> lane) & 1ull;
+            const int na = popc(am);
+            if constexpr (AUX != 0) S.last_acc += na;
             const int pos_a = r_below + a_below;
             const int total = rn + na;
             const int nrn = min(ef, total);
