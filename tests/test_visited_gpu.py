@@ -60,6 +60,13 @@ def m16d128_ip():
 
 
 @pytest.fixture(scope="module")
+def m16d120():
+    """6,000 x 120, M = 16: rows padded to 128 bytes, so the constant shape applies with zero
+    padding in the last 8 bytes of every row and query."""
+    return _build(6_000, 120, 16, 43)
+
+
+@pytest.fixture(scope="module")
 def wide_ids():
     """70,000 x 16, M = 8: 17-bit ids, so a 64-slot table needs 32-bit slots."""
     return _build(70_000, 16, 8, 41)
@@ -162,6 +169,22 @@ def test_two_phase_inner_product_matches_oracle(m16d128_ip, ef: int) -> None:
         _tune(eng)
     _same(two, want, f"ip two-phase ef={ef}")
     _same(eng.search_many(queries, 10, ef), want, f"ip ef={ef}")
+
+
+@pytest.mark.parametrize("ef", [48, 100])
+def test_padded_rows_match_oracle(m16d120, ef: int) -> None:
+    """120 dimensions on the constant shape, in one launch and after the batched descent."""
+    eng, _, queries = m16d120
+    queries = queries[:397]
+    want = _oracle(eng, queries, 10, ef)
+    print(f"case m16d120 ef={ef}", flush=True)
+    _same(eng.search_many(queries, 10, ef), want, f"d120 ef={ef}")
+    try:
+        _tune(eng, waves_per_cu=1)
+        two = eng.search_many(queries, 10, ef)
+    finally:
+        _tune(eng)
+    _same(two, want, f"d120 two-phase ef={ef}")
 
 
 def test_wide_beam_on_the_bitset_matches_oracle(m16d128) -> None:
