@@ -222,13 +222,42 @@ HM_DEV void exact_rows(const SearchArgs &a, const Lds &L, int nnew, int lg_rt) {
 // Exact integer distances of the nnew ids in L.nbi to the 8-bit query. Lanes 0..nnew-1 get
 // their key. Every partial sum is an exact integer, so any summation order equals the
 // reference's float64 sum.
+// Constant shape (128-byte rows): 8 lanes per row, 8 rows per pass, at most 4 passes; each
+// pass's row sums reach lanes 0..nnew-1 by ds_bpermute (lane i takes row i's sum from lane
+// 8 (i % 8) of pass i / 8), with no round trip through LDS memory.
+HM_DEV uint64_t dist_exact_rows128(const SearchArgs &a, const Lds &L, int nnew, int32_t qq) {
+    const int lane = lane_id();
+    const int c = lane & 7;
+    const uint4 q4 = ((const uint4 *)L.qb)[c];
+    uint32_t ids[4];
+#pragma unroll
+    for (int it = 0; it < 4; ++it) ids[it] = L.nbi[min(it * 8 + (lane >> 3), nnew - 1)];
+    uint4 xs[4];
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        if (it * 8 < nnew) xs[it] = *(const uint4 *)(a.vec_u8 + ((size_t)ids[it] << 7) + c * 16);
+    }
+    const int src = c << 5;  // byte address of lane 8 (lane % 8)
+    int32_t s = 0;
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        if (it * 8 >= nnew) break;
+        const int32_t row = group_sum(exact_partial(xs[it], q4, a.metric), 8);
+        const int32_t got = __builtin_amdgcn_ds_bpermute(src, row);
+        if ((lane >> 3) == it) s = got;
+    }
+    uint64_t key = kInfKey;
+    if (lane < nnew) key = a.metric == kL2 ? key_of_exact(s + qq, kL2) : key_of_exact(-s, a.metric);
+    return key;
+}
+
 template <int SH>
 HM_DEV uint64_t dist_exact(const SearchArgs &a, const Lds &L, int nnew, int32_t qq) {
     const int lane = lane_id();
     const int cpv = a.dimp >> 4;
     const uint4 *qv = (const uint4 *)L.qb;
     if constexpr (SH != kShapeAny) {
-        exact_rows<3>(a, L, nnew, 3);
+        return dist_exact_rows128(a, L, nnew, qq);
     } else if ((cpv & (cpv - 1)) == 0 && cpv <= 64) {
         exact_rows<-1>(a, L, nnew, __builtin_ctz(cpv));
     } else {
