@@ -17,7 +17,7 @@ ef, k, |R|/ef, unexpanded entries/ef, signed log2 of the best / k-th / next cand
 distance, (next - k-th)/(|k-th - best| + 1), (k-th - best)/(|best| + 1),
 (worst - k-th)/(|k-th - best| + 1), expansions since the k-th
 changed, accepted and new neighbours of the last expansion, entries of the current top-k not
-yet expanded. The search evaluates the model after every fourth expansion.
+yet expanded. The search evaluates the model after every eighth expansion.
 """
 
 from __future__ import annotations
