@@ -1701,12 +1701,19 @@ __global__ __launch_bounds__(1024) void order_by_entry_kernel(const uint64_t *ke
     };
     for (int i = t; i < nq; i += 1024) atomicAdd(&count[bucket(key[i])], 1);
     __syncthreads();
-    if (t == 0) {
-        int s = 0;
-        for (int b = 0; b < 256; ++b) {
-            start[b] = s;
-            s += count[b];
+    if (t < 64) {  // exclusive prefix of the 256 counts: 4 per lane, then a scan over the wave
+        const int c0 = count[4 * t], c1 = count[4 * t + 1], c2 = count[4 * t + 2];
+        const int own = c0 + c1 + c2 + count[4 * t + 3];
+        int incl = own;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = shfl(incl, t >= o ? t - o : t);
+            if (t >= o) incl += v;
         }
+        const int excl = incl - own;
+        start[4 * t] = excl;
+        start[4 * t + 1] = excl + c0;
+        start[4 * t + 2] = excl + c0 + c1;
+        start[4 * t + 3] = excl + c0 + c1 + c2;
     }
     __syncthreads();
     for (int i = t; i < nq; i += 1024) order[atomicAdd(&start[bucket(key[i])], 1)] = i;
