@@ -1687,8 +1687,11 @@ __global__ void reset_queue_kernel(int32_t *q) { q[0] = 0; }
 // work queue of the layer-0 launch that follows.
 __global__ __launch_bounds__(1024) void order_by_entry_kernel(const uint64_t *key, int nq,
                                                               int32_t *order, int32_t *queue) {
+    constexpr int kKeep = 16384;  // batches up to this size keep each query's bucket in LDS
     __shared__ int32_t count[256];
     __shared__ int32_t start[256];
+    __shared__ uint8_t kept[kKeep];
+    const bool keep = nq <= kKeep;
     const int t = threadIdx.x;
     if (t == 0) queue[0] = 0;
     if (t < 256) count[t] = 0;
@@ -1699,7 +1702,11 @@ __global__ __launch_bounds__(1024) void order_by_entry_kernel(const uint64_t *ke
         const int frac = e >= 3 ? (int)((v >> (e - 3)) & 7u) : (int)((v << (3 - e)) & 7u);
         return 255 - min(255, e * 8 + frac);  // descending key -> ascending bucket
     };
-    for (int i = t; i < nq; i += 1024) atomicAdd(&count[bucket(key[i])], 1);
+    for (int i = t; i < nq; i += 1024) {
+        const int b = bucket(key[i]);
+        if (keep) kept[i] = (uint8_t)b;
+        atomicAdd(&count[b], 1);
+    }
     __syncthreads();
     if (t < 64) {  // exclusive prefix of the 256 counts: 4 per lane, then a scan over the wave
         const int c0 = count[4 * t], c1 = count[4 * t + 1], c2 = count[4 * t + 2];
@@ -1716,7 +1723,10 @@ __global__ __launch_bounds__(1024) void order_by_entry_kernel(const uint64_t *ke
         start[4 * t + 3] = excl + c0 + c1 + c2;
     }
     __syncthreads();
-    for (int i = t; i < nq; i += 1024) order[atomicAdd(&start[bucket(key[i])], 1)] = i;
+    for (int i = t; i < nq; i += 1024) {
+        const int b = keep ? (int)kept[i] : bucket(key[i]);
+        order[atomicAdd(&start[b], 1)] = i;
+    }
 }
 
 // Upper-layer descent of a whole batch at once (phase 1 of the two-phase schedule) for up to
