@@ -119,17 +119,12 @@ def quantized(log=print) -> dict:
             "quantizer": {"step": float(quant.step)}, "exact_seconds": t_exact, "rows": rows}
 
 
-def mixed(log=print) -> dict:
-    """One persistent launch over 100,000 mixed queries on the 1M x 128 index: ef drawn from
-    {16, 32, 64, 128, 256, 512}, k from {1, 10, 100}, half of the queries with an i.i.d. mask of
-    selectivity 5-50 % (a pool of 8 masks). Each query follows exactly the reference semantics
-    of its own arguments."""
+def mixed_batch(w, nq: int = 100_000):
+    """The mixed batch on workload ``w`` (fixed seeds): ``(queries, k, ef, pool, mask_index)``
+    with ef drawn from {16, 32, 64, 128, 256, 512} (raised to k), k from {1, 10, 100}, and half
+    of the queries carrying one of 8 i.i.d. masks of selectivity 5-50 % (-1: unfiltered)."""
     from .synthetic import clustered_u8_queries, random_mask
-    from .workloads import WORKLOADS, load_or_build
 
-    w = WORKLOADS["clustered_1m"]
-    eng, data, _ = load_or_build(w, log=log)
-    nq = 100_000
     queries = clustered_u8_queries(nq, w.dim, n_clusters=w.n_clusters, sigma=w.sigma,
                                    seed=w.seed, query_seed=w.seed + 3)
     rng = np.random.default_rng(5)
@@ -138,8 +133,20 @@ def mixed(log=print) -> dict:
     sels = np.linspace(0.05, 0.5, 8)
     pool = np.stack([random_mask(w.n, s, 300 + i) for i, s in enumerate(sels)])
     midx = np.where(rng.random(nq) < 0.5, rng.integers(0, len(sels), size=nq), -1)
-    t, got = _best(lambda: eng.search_many(queries, k, np.maximum(ef, k), mask_pool=pool,
-                                           mask_index=midx), 2)
+    return queries, k, np.maximum(ef, k), pool, midx
+
+
+def mixed(log=print) -> dict:
+    """One persistent launch over 100,000 mixed queries on the 1M x 128 index
+    (:func:`mixed_batch`). Each query follows exactly the reference semantics of its own
+    arguments."""
+    from .workloads import WORKLOADS, load_or_build
+
+    w = WORKLOADS["clustered_1m"]
+    eng, data, _ = load_or_build(w, log=log)
+    queries, k, ef, pool, midx = mixed_batch(w)
+    nq = queries.shape[0]
+    t, got = _best(lambda: eng.search_many(queries, k, ef, mask_pool=pool, mask_index=midx), 2)
     plain = midx < 0
     return {"workload": f"{w.name}: {nq} mixed queries (ef 16-512, k 1/10/100, "
                         f"{int((~plain).sum())} filtered at 5-50 %)",

@@ -344,6 +344,14 @@ class HNSWIndex:
             distance_computations=self._engine.distance_computations, hops=self._engine.hops,
         )
 
+    @property
+    def distance_computations(self) -> int:
+        return self._engine.distance_computations
+
+    @property
+    def hops(self) -> int:
+        return self._engine.hops
+
     def reset_counters(self) -> None:
         self._engine.distance_computations = 0
         self._engine.hops = 0

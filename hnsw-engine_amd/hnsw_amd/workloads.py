@@ -99,6 +99,13 @@ def _digest(data: np.ndarray, levels: np.ndarray, w: Workload) -> str:
 GRAPH_ARRAYS = ("levels", "links0", "count0", "upper_offset", "links_up", "count_up")
 
 
+def is_cached(w: Workload) -> bool:
+    """True when the graph of ``w`` is in the cache (it may still be stale: ``load_or_build``
+    compares the digest)."""
+    header = CACHE / w.name / "header.json"
+    return header.is_file() and json.loads(header.read_text()).get("workload", {}).get("n") == w.n
+
+
 def load_or_build(w: Workload, *, log=print) -> tuple[MI355XEngine, np.ndarray, np.ndarray]:
     """``(engine, data, queries)`` with the graph loaded from the cache or built (and cached)."""
     t0 = time.perf_counter()
@@ -129,3 +136,10 @@ def load_or_build(w: Workload, *, log=print) -> tuple[MI355XEngine, np.ndarray, 
     header.write_text(json.dumps({"digest": digest, "entry_point": eng.entry_point,
                                   "max_level": eng.max_level, "workload": w.__dict__}, indent=2))
     return eng, data, queries
+
+
+if __name__ == "__main__":  # python -m hnsw_amd.workloads <name> ...: build and cache graphs
+    import sys
+
+    for name in sys.argv[1:] or list(WORKLOADS):
+        load_or_build(WORKLOADS[name], log=lambda m: print(m, flush=True))
