@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -59,6 +60,32 @@ struct Batch {
 };
 
 constexpr int64_t kDefaultSpill = 16384;
+
+// Scratch of hm_search_dev_noalloc, for callers that cannot allocate: 64 MiB per GPU, held from
+// the library's load until the process ends. Host code never passes this name to a kernel (on
+// the host it is only a stand-in for the symbol): hm_open looks up each device's address of it
+// once, and the entry point carves its scratch from that address.
+constexpr size_t kStaticScratchBytes = size_t(64) << 20;
+__device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (current), looked up once per device.
+hipError_t static_scratch_of(int device, void **out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
 // Ids per wave slot remembered for clearing its visited bitset (a layer search that sets more
 // clears the whole bitset).
 constexpr int kVisitedList = 4096;
@@ -90,8 +117,37 @@ void plan(SearchArgs &a, int mode, const hm_index *h, int max_ef, bool fast, int
         lds_bytes_per_wave(mode, h->dim, h->dimp, a.r_cap, a.c_cap, a.h_log2, a.id_bits);
 }
 
-int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, int max_slots,
-           hipStream_t s, bool allow_fast = true) {
+// Where a launch's scratch comes from: the handle's grow-only buffers (allocated on demand), or
+// a fixed region carved in order (no allocation; the launch shrinks its wave slots to fit).
+struct Carver {
+    uint8_t *base = nullptr;
+    size_t cap = 0, used = 0;
+    template <class T> T *take(size_t count) {
+        const size_t at = (used + 255) & ~size_t(255);
+        if (at + count * sizeof(T) > cap) return nullptr;
+        used = at + count * sizeof(T);
+        return (T *)(base + at);
+    }
+};
+
+// Zeroes `words` uint32 words (a kernel, not a memset node, so that captured work stays
+// correct on replay).
+__global__ void clear_words_kernel(uint32_t *p, int64_t words) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < words;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+bool two_phase_shape(const hm_index *h, int mode) {
+    return mode == kExactU8 && h->w1 == 16 && h->dimp == 128 && h->dim <= 128;
+}
+
+// Launches one batch. fixed == null: scratch from the handle's buffers; otherwise every scratch
+// array (and the counters / fail flags the batch leaves null) is carved from *fixed, the
+// bitsets and event counters cleared by a kernel first, and the wave slots reduced to what fits.
+int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap, int max_slots,
+           hipStream_t s, bool allow_fast = true, Carver *fixed = nullptr) {
+    Batch b = b0;
     SearchArgs a;
     memset(&a, 0, sizeof(a));
     a.links0 = h->links0.p;
@@ -121,11 +177,6 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, 
     a.mask_words = b.mask_words;
     a.mask_idx = b.midx;
     a.mask_all = b.mask_all;
-    a.out_ids = b.ids;
-    a.out_d = b.dists;
-    a.out_dc = b.dc;
-    a.out_hops = b.hops;
-    a.out_fail = b.fail;
     const bool fast = allow_fast && !filt && b.max_ef <= 64 && b.kstride <= 64 && mode != kSeqU8;
     // ef 65..512 on the constant shape: the register beam with two, four or eight entries per
     // lane.
@@ -156,46 +207,122 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b, int64_t spill_cap, 
     if (a.lds_per_wave > 160 * 1024) return invalid("ef too large for the on-chip search state");
     int slots = search_slots(mode, filt, a, a.lds_per_wave, h->cus);
     if (h->tune_waves_per_cu > 0) slots = std::min(slots, h->tune_waves_per_cu * h->cus);
-    HM_TRY(h->stats.ensure(2, true, s), "stats allocation");
-    a.stats = h->stats.p;
     slots = (int)std::min<int64_t>(slots, std::max<int64_t>(1, b.nq));
     if (max_slots > 0) slots = std::min(slots, max_slots);
-    a.slots = slots;
     a.bitset_words = (h->n + 31) / 32;
-    HM_TRY(h->bits.ensure((size_t)slots * a.bitset_words, true, s), "bitset allocation");
-    a.bitsets = h->bits.p;
     a.vlist_cap = kVisitedList;
-    HM_TRY(h->vlist.ensure((size_t)slots * kVisitedList), "visited list allocation");
-    a.vlist = h->vlist.p;
+    // The register beams never spill candidates.
+    if (a.fast) spill_cap = 0;
     a.spill_cap = spill_cap;
-    HM_TRY(h->spill_key.ensure((size_t)slots * spill_cap), "spill allocation");
-    HM_TRY(h->spill_id.ensure((size_t)slots * spill_cap), "spill allocation");
-    a.spill_key = h->spill_key.p;
-    a.spill_id = h->spill_id.p;
-    HM_TRY(h->queue.ensure(1), "queue allocation");
-    a.queue = h->queue.p;
+    if (fixed) {
+        Carver &c = *fixed;
+        int32_t *queue = c.take<int32_t>(1);
+        int32_t *stats = c.take<int32_t>(2);
+        if (!b.dc) b.dc = c.take<int32_t>(b.nq);
+        if (!b.hops) b.hops = c.take<int32_t>(b.nq);
+        if (!b.fail) b.fail = c.take<int32_t>(b.nq);
+        // The schedule arrays are reserved whenever the batch might take the two-phase schedule.
+        uint64_t *pre_key = nullptr;
+        uint32_t *pre_id = nullptr;
+        int32_t *pre_dc = nullptr, *pre_hops = nullptr, *order = nullptr;
+        if (a.fast && two_phase_shape(h, mode) && !a.trace && h->tune_schedule == 0) {
+            pre_key = c.take<uint64_t>(b.nq);
+            pre_id = c.take<uint32_t>(b.nq);
+            pre_dc = c.take<int32_t>(b.nq);
+            pre_hops = c.take<int32_t>(b.nq);
+            order = c.take<int32_t>(b.nq);
+            if (!order) return invalid("batch too large for the allocation-free scratch");
+        }
+        if (!queue || !stats || !b.dc || !b.hops || !b.fail)
+            return invalid("batch too large for the allocation-free scratch");
+        const size_t per_slot = (size_t)a.bitset_words * 4 + (size_t)kVisitedList * 4 +
+                                (size_t)spill_cap * 12 + 3 * 256;
+        const size_t room = c.cap > c.used + 256 ? c.cap - c.used - 256 : 0;
+        const int fit = (int)std::min<size_t>(room / per_slot, (size_t)INT32_MAX);
+        if (fit < 1) return invalid("graph too large for the allocation-free scratch");
+        slots = std::min(slots, fit);
+        a.bitsets = c.take<uint32_t>((size_t)slots * a.bitset_words);
+        a.vlist = c.take<uint32_t>((size_t)slots * kVisitedList);
+        a.spill_key = c.take<uint64_t>(std::max<size_t>(1, (size_t)slots * spill_cap));
+        a.spill_id = c.take<uint32_t>(std::max<size_t>(1, (size_t)slots * spill_cap));
+        if (!a.bitsets || !a.vlist || !a.spill_key || !a.spill_id)
+            return invalid("graph too large for the allocation-free scratch");
+        a.queue = queue;
+        a.stats = stats;
+        a.pre_key = pre_key;
+        a.pre_id = pre_id;
+        a.pre_dc = pre_dc;
+        a.pre_hops = pre_hops;
+        a.order = order;
+        hipLaunchKernelGGL(clear_words_kernel, dim3(1), dim3(64), 0, s, (uint32_t *)stats,
+                           (int64_t)2);
+        const int64_t words = (int64_t)slots * a.bitset_words;
+        const int grid = (int)std::min<int64_t>(4096, (words + 1023) / 1024);
+        hipLaunchKernelGGL(clear_words_kernel, dim3(std::max(1, grid)), dim3(256), 0, s,
+                           a.bitsets, words);
+    } else {
+        HM_TRY(h->stats.ensure(2, true, s), "stats allocation");
+        a.stats = h->stats.p;
+        HM_TRY(h->bits.ensure((size_t)slots * a.bitset_words, true, s), "bitset allocation");
+        a.bitsets = h->bits.p;
+        HM_TRY(h->vlist.ensure((size_t)slots * kVisitedList), "visited list allocation");
+        a.vlist = h->vlist.p;
+        HM_TRY(h->spill_key.ensure((size_t)slots * spill_cap), "spill allocation");
+        HM_TRY(h->spill_id.ensure((size_t)slots * spill_cap), "spill allocation");
+        a.spill_key = h->spill_key.p;
+        a.spill_id = h->spill_id.p;
+        HM_TRY(h->queue.ensure(1), "queue allocation");
+        a.queue = h->queue.p;
+    }
+    a.out_ids = b.ids;
+    a.out_d = b.dists;
+    a.out_dc = b.dc;
+    a.out_hops = b.hops;
+    a.out_fail = b.fail;
+    a.slots = slots;
     // More queries than wave slots on the register beam (128 one-byte dimensions, 16 upper
     // links): every query's upper-layer descent first, 16 lanes per query (descend_kernel), then
     // layer 0 with the queries whose layer-0 entry lies farthest from them first, which
     // shortens the launch's tail. Measured on 10,000 queries over 1M x 128 against one launch:
     // ef 32 / 48 / 64 / 96 -12 / -17 / -16 / -14 %, ids identical.
-    const bool two_shape = mode == kExactU8 && h->w1 == 16 && h->dimp == 128 && h->dim <= 128;
-    if (a.fast && two_shape && !a.trace && b.nq > slots && h->tune_schedule == 0) {
+    if (a.fast && two_phase_shape(h, mode) && !a.trace && b.nq > slots && h->tune_schedule == 0) {
         a.descent_cap = h->tune_descent_cap > 0 ? h->tune_descent_cap : kDescentMaxCap;
-        HM_TRY(h->pre_key.ensure(b.nq), "schedule allocation");
-        HM_TRY(h->pre_id.ensure(b.nq), "schedule allocation");
-        HM_TRY(h->pre_dc.ensure(b.nq), "schedule allocation");
-        HM_TRY(h->pre_hops.ensure(b.nq), "schedule allocation");
-        HM_TRY(h->order.ensure(b.nq), "schedule allocation");
+        if (!fixed) {
+            HM_TRY(h->pre_key.ensure(b.nq), "schedule allocation");
+            HM_TRY(h->pre_id.ensure(b.nq), "schedule allocation");
+            HM_TRY(h->pre_dc.ensure(b.nq), "schedule allocation");
+            HM_TRY(h->pre_hops.ensure(b.nq), "schedule allocation");
+            HM_TRY(h->order.ensure(b.nq), "schedule allocation");
+            a.pre_key = h->pre_key.p;
+            a.pre_id = h->pre_id.p;
+            a.pre_dc = h->pre_dc.p;
+            a.pre_hops = h->pre_hops.p;
+            a.order = h->order.p;
+        }
         a.phase = 2;  // search_launch runs phase 1, the order, then phase 2
-        a.pre_key = h->pre_key.p;
-        a.pre_id = h->pre_id.p;
-        a.pre_dc = h->pre_dc.p;
-        a.pre_hops = h->pre_hops.p;
-        a.order = h->order.p;
+    } else {
+        a.order = nullptr;  // carved schedule arrays stay unused
     }
     HM_TRY(search_launch(mode, filt, a, slots, s), "search launch");
     return HM_OK;
+}
+
+// Calls on one handle share its scratch, so each call's stream first waits for the work the
+// handle's previous call enqueued (on whatever stream), and records the handle's event after its
+// own. Inside a stream capture neither is done: the captured launches are ordered by the stream
+// the graph is later launched on, and hm_search_graph_dev orders the launch itself.
+hipError_t order_before(hm_index *h, hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone)
+        return hipSuccess;
+    return h->last_done ? hipStreamWaitEvent(s, h->last_done, 0) : hipSuccess;
+}
+
+hipError_t order_after(hm_index *h, hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone)
+        return hipSuccess;
+    return h->last_done ? hipEventRecord(h->last_done, s) : hipSuccess;
 }
 
 // Packs boolean rows (n bytes each) into uint32 bitsets.
@@ -227,6 +354,7 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
     if (nq == 0) return HM_OK;
     hipStream_t s = h->stream;
     HM_TRY(hipSetDevice(h->device), "hipSetDevice");
+    HM_TRY(order_before(h, s), "stream order");
     Batch b;
     b.nq = nq;
     b.kstride = o->kstride;
@@ -336,6 +464,7 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
     HM_TRY(hipMemcpyAsync(dists, h->dists.p, on * 8, hipMemcpyDeviceToHost, s), "copy dists");
     HM_TRY(hipMemcpyAsync(dcv.data(), h->dc.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
     HM_TRY(hipMemcpyAsync(hv.data(), h->hops.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
+    HM_TRY(order_after(h, s), "stream order");
     HM_TRY(hipStreamSynchronize(s), "search");
     for (int64_t i = 0; i < nq; ++i) {
         if (dcount) dcount[i] = dcv[i];
@@ -392,6 +521,11 @@ int hm_open(const hm_native_graph *g, int device, hm_index **out) {
     };
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(fail_hip(e, "stream"));
+    e = hipEventCreateWithFlags(&h->last_done, hipEventDisableTiming);
+    if (e != hipSuccess) return fail(fail_hip(e, "event"));
+    // Only hm_search_dev_noalloc needs the static scratch; the index opens without it.
+    e = static_scratch_of(device, &h->static_scratch);
+    if (e != hipSuccess) h->static_scratch_error = hipGetErrorString(e);
     const int64_t n = g->n;
     // Layer-0 rows: the used slots in stored order, repeats dropped, -1 beyond.
     {
@@ -461,6 +595,7 @@ void hm_close(hm_index *h) {
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     h->release_all();
+    if (h->last_done) (void)hipEventDestroy(h->last_done);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -484,7 +619,7 @@ namespace {
 // The batch of hm_search_dev / hm_search_graph_dev (arguments already checked).
 int device_batch(hm_index *h, const float *d_queries, const hm_search_opts *o,
                  const uint32_t *d_mask_bits, int64_t *d_ids, double *d_dists, int32_t *d_dcount,
-                 int32_t *d_hops, int32_t *d_fail, Batch &b, bool &filt) {
+                 int32_t *d_hops, int32_t *d_fail, Batch &b, bool &filt, bool alloc = true) {
     b = Batch();
     b.nq = o->nq;
     b.q32 = d_queries;
@@ -497,14 +632,18 @@ int device_batch(hm_index *h, const float *d_queries, const hm_search_opts *o,
     b.midx = o->mask_index;
     b.mask_all = o->mask_all;
     filt = d_mask_bits != nullptr && (o->mask_index || o->mask_all >= 0);
+    b.ids = d_ids;
+    b.dists = d_dists;
+    b.dc = d_dcount;
+    b.hops = d_hops;
+    b.fail = d_fail;
+    if (!alloc) return HM_OK;  // the allocation-free path carves what is null
     HM_TRY(h->dc.ensure(o->nq), "alloc");
     HM_TRY(h->hops.ensure(o->nq), "alloc");
     HM_TRY(h->fail.ensure(o->nq), "alloc");
-    b.ids = d_ids;
-    b.dists = d_dists;
-    b.dc = d_dcount ? d_dcount : h->dc.p;
-    b.hops = d_hops ? d_hops : h->hops.p;
-    b.fail = d_fail ? d_fail : h->fail.p;
+    if (!b.dc) b.dc = h->dc.p;
+    if (!b.hops) b.hops = h->hops.p;
+    if (!b.fail) b.fail = h->fail.p;
     b.learned = o->learned != 0;
     return HM_OK;
 }
@@ -552,8 +691,39 @@ int hm_search_dev(hm_index *h, const float *d_queries, const hm_search_opts *o,
     st = device_batch(h, d_queries, o, d_mask_bits, d_ids, d_dists, d_dcount, d_hops, d_fail, b,
                       filt);
     if (st) return st;
-    return launch(h, exact_u8 ? kExactU8 : kSeqF32, filt, b, std::min<int64_t>(h->n, kDefaultSpill),
-                  0, (hipStream_t)stream);
+    hipStream_t s = (hipStream_t)stream;
+    HM_TRY(order_before(h, s), "stream order");
+    st = launch(h, exact_u8 ? kExactU8 : kSeqF32, filt, b, std::min<int64_t>(h->n, kDefaultSpill),
+                0, s);
+    if (st) return st;
+    HM_TRY(order_after(h, s), "stream order");
+    return HM_OK;
+}
+
+int hm_search_dev_noalloc(hm_index *h, const float *d_queries, const hm_search_opts *o,
+                          const uint32_t *d_mask_bits, int exact_u8, int64_t *d_ids,
+                          double *d_dists, int32_t *d_dcount, int32_t *d_hops, int32_t *d_fail,
+                          void *stream) {
+    int st = check_dev(h, d_queries, o, exact_u8, d_ids, d_dists);
+    if (st || o->nq == 0) return st;
+    if (!h->static_scratch)
+        return invalid("the allocation-free scratch is unavailable: " + h->static_scratch_error);
+    HM_TRY(hipSetDevice(h->device), "hipSetDevice");
+    Batch b;
+    bool filt = false;
+    st = device_batch(h, d_queries, o, d_mask_bits, d_ids, d_dists, d_dcount, d_hops, d_fail, b,
+                      filt, false);
+    if (st) return st;
+    hipStream_t s = (hipStream_t)stream;
+    HM_TRY(order_before(h, s), "stream order");
+    Carver c;
+    c.base = (uint8_t *)h->static_scratch;
+    c.cap = kStaticScratchBytes;
+    st = launch(h, exact_u8 ? kExactU8 : kSeqF32, filt, b, std::min<int64_t>(h->n, kDefaultSpill),
+                0, s, true, &c);
+    if (st) return st;
+    HM_TRY(order_after(h, s), "stream order");
+    return HM_OK;
 }
 
 int hm_search_graph_dev(hm_index *h, const float *d_queries, const hm_search_opts *o,
@@ -570,8 +740,10 @@ int hm_search_graph_dev(hm_index *h, const float *d_queries, const hm_search_opt
     st = device_batch(h, d_queries, o, d_mask_bits, d_ids, d_dists, d_dcount, d_hops, d_fail, b,
                       filt);
     if (st) return st;
+    HM_TRY(order_before(h, s), "stream order");
     if (h->graph_exec && graph_key(h, d_queries, o, d_mask_bits, exact_u8, b) == h->graph_key) {
         HM_TRY(hipGraphLaunch(h->graph_exec, s), "graph launch");
+        HM_TRY(order_after(h, s), "stream order");
         return HM_OK;
     }
     // A new shape: run it directly (this allocates whatever scratch it lacks), then capture the
@@ -603,6 +775,7 @@ int hm_search_graph_dev(hm_index *h, const float *d_queries, const hm_search_opt
     if (ei != hipSuccess) return fail_hip(ei, "graph instantiate");
     h->graph_exec = x;
     h->graph_key = key;
+    HM_TRY(order_after(h, s), "stream order");
     return HM_OK;
 }
 
@@ -637,8 +810,13 @@ int hm_trace_dev(hm_index *h, const float *d_queries, int64_t nq, int32_t k, int
     b.trace_thr = d_thr;
     b.trace_len = d_len;
     b.trace_final = d_final;
-    return launch(h, exact_u8 ? kExactU8 : kSeqF32, false, b, std::min<int64_t>(h->n, kDefaultSpill),
-                  0, (hipStream_t)stream);
+    hipStream_t s = (hipStream_t)stream;
+    HM_TRY(order_before(h, s), "stream order");
+    int st = launch(h, exact_u8 ? kExactU8 : kSeqF32, false, b,
+                    std::min<int64_t>(h->n, kDefaultSpill), 0, s);
+    if (st) return st;
+    HM_TRY(order_after(h, s), "stream order");
+    return HM_OK;
 }
 
 int hm_trace(hm_index *h, const float *queries, int64_t nq, int32_t k, int32_t ef,

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -15,6 +16,12 @@ struct hm_index {
     int cus = 1;
     bool exact_u8 = false, has_codes = false;
     hipStream_t stream = nullptr;
+    // Recorded after each call's work; the next call's stream waits on it (calls share scratch).
+    hipEvent_t last_done = nullptr;
+    // This device's copy of the allocation-free entry point's scratch array (null: unavailable,
+    // static_scratch_error says why).
+    void *static_scratch = nullptr;
+    std::string static_scratch_error;
     // graph: links0 (n, w0), links_up (rows, w1), upper (n,) first upper row, -1 beyond counts
     hm::DevBuf<int32_t> links0, links_up, upper;
     hm::DevBuf<float> vec;        // (n, dim)

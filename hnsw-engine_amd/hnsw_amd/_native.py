@@ -21,6 +21,7 @@ SYMBOLS = (
     "hm_search",
     "hm_search_dev",
     "hm_search_graph_dev",
+    "hm_search_dev_noalloc",
     "hm_queries_exact",
     "hm_attach_codes",
     "hm_search_codes",
@@ -110,6 +111,10 @@ def lib() -> ctypes.CDLL:
                 ctypes.c_int,
             ),
             "hm_search_graph_dev": (
+                [vp, vp, ctypes.POINTER(SearchOpts), vp, ctypes.c_int, vp, vp, vp, vp, vp, vp],
+                ctypes.c_int,
+            ),
+            "hm_search_dev_noalloc": (
                 [vp, vp, ctypes.POINTER(SearchOpts), vp, ctypes.c_int, vp, vp, vp, vp, vp, vp],
                 ctypes.c_int,
             ),

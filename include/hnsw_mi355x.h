@@ -5,7 +5,10 @@
  * status; hm_last_error() then holds the message (the Python layer re-raises it as the
  * reference's exception type). Host-pointer entry points copy to and from HBM themselves;
  * the *_dev variants take device pointers and a HIP stream; the search ones allocate nothing
- * per call beyond the scratch the handle keeps between calls.
+ * per call beyond the scratch the handle keeps between calls. Calls on one handle share that
+ * scratch: each search call's stream waits on the device for the handle's previous call (a
+ * HIP event the handle records after each call), so calls on different streams never overlap
+ * on one handle.
  *
  * Reference interfaces replaced (paths relative to the hnsw-engine repository):
  *   hm_open            NumbaEngine.attach_native      src/hnsw_engine/numba_engine.py:827-868
@@ -123,6 +126,23 @@ int hm_search_graph_dev(hm_index *index, const float *d_queries, const hm_search
                         const uint32_t *d_mask_bits, int exact_u8, int64_t *d_ids,
                         double *d_dists, int32_t *d_dcount, int32_t *d_hops, int32_t *d_fail,
                         void *stream);
+
+/* hm_search_dev for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). No counterpart in the reference. Same
+ * arguments and results as hm_search_dev; it calls no hipMalloc / hipFree. Its scratch (work
+ * queue, visited bitsets, clearing lists, candidate spill, schedule arrays, and the dcount / hops
+ * / fail arrays the caller leaves null) is carved from one static 64 MiB device array per GPU,
+ * held from the library's load to the end of the process: the batch runs on as many wave slots
+ * as that array holds (about 470 at 1M nodes, so it is slower than hm_search_dev on large
+ * graphs), and a batch or graph too large for it is refused with HM_EINVAL. The array is shared
+ * by every call on that GPU, from any handle: calls that overlap on one GPU race (calls on one
+ * handle are ordered, see below); each call clears what must start at zero. Ordering: every
+ * call on a handle waits (on the device) for the work its previous call on that handle
+ * enqueued, whatever the streams. */
+int hm_search_dev_noalloc(hm_index *index, const float *d_queries, const hm_search_opts *opts,
+                          const uint32_t *d_mask_bits, int exact_u8, int64_t *d_ids,
+                          double *d_dists, int32_t *d_dcount, int32_t *d_hops, int32_t *d_fail,
+                          void *stream);
 
 /* 1 if every value of the host batch is an integer in 0..255 (exact 8-bit kernel eligible). */
 int hm_queries_exact(const float *queries, int64_t count);
