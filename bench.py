@@ -69,7 +69,13 @@ def main() -> int:
     ap.add_argument("--train-queries", type=int, default=10_000)
     ap.add_argument("--learned-ef", default="56,64")
     ap.add_argument("--thresholds", default="0.6,0.7,0.8,0.85,0.9,0.95")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB on the device three times (run under rocprofv3 --pmc "
+                         "FETCH_SIZE / WRITE_SIZE: 1,048,576 KiB over the counter's KiB per "
+                         "dispatch is the factor that turns the counter into bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(
             f"bench.py measures one GPU only (asked for {args.gpus}); the search path shards by "
@@ -312,6 +318,18 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def copy_calibration() -> int:
+    import torch
+
+    src = torch.ones(1 << 28, device="cuda")  # float32: 1 GiB
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"copied 3 x {src.numel() * 4} bytes")
     return 0
 
 

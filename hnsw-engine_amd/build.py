@@ -26,7 +26,7 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> Path:
-    sources = sorted(str(p) for p in CSRC.glob("*.hip")) + [str(CSRC / "builder.cpp")]
+    sources = sorted(str(p) for p in CSRC.glob("*.hip")) + sorted(str(p) for p in CSRC.glob("*.cpp"))
     tmp = OUT.with_suffix(".so.tmp")
     cmd = [
         hipcc(),

@@ -16,6 +16,8 @@
 namespace hm {
 
 int search_slots(int mode, bool filt, const SearchArgs &a, int lds, int device_cus);
+hipError_t rerank_device(hm_index *h, const float *d_q, const int64_t *d_cand, int64_t nq,
+                         int fetch, int k, int64_t *d_ids, double *d_d, hipStream_t s);
 hipError_t search_launch(int mode, bool filt, const SearchArgs &a, int blocks, hipStream_t s);
 
 static thread_local std::string g_error;
@@ -344,10 +346,12 @@ int check_opts(const hm_index *h, const hm_search_opts *o) {
     return HM_OK;
 }
 
-// Host-buffer search shared by hm_search and hm_search_codes.
+// Host-buffer search shared by hm_search, hm_search_codes and hm_search_codes_rerank. With
+// rerank_q (prepared float32 queries, host) the batch's kstride candidates stay in HBM and the
+// exact re-rank scores them there; ids / dists then receive its best rerank_k per query.
 int search_host(hm_index *h, int mode, const float *q32, const double *q64,
                 const hm_search_opts *o, int64_t *ids, double *dists, int64_t *dcount,
-                int64_t *hops) {
+                int64_t *hops, const float *rerank_q = nullptr, int rerank_k = 0) {
     int st = check_opts(h, o);
     if (st) return st;
     const int64_t nq = o->nq;
@@ -460,8 +464,23 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
         for (int64_t i = 0; i < nq; ++i)
             if (failv[i]) return invalid("candidate spill overflow");
     }
-    HM_TRY(hipMemcpyAsync(ids, h->ids.p, on * 8, hipMemcpyDeviceToHost, s), "copy ids");
-    HM_TRY(hipMemcpyAsync(dists, h->dists.p, on * 8, hipMemcpyDeviceToHost, s), "copy dists");
+    if (rerank_q) {
+        const size_t rn = (size_t)nq * rerank_k;
+        HM_TRY(h->q32.ensure((size_t)nq * h->dim), "alloc");
+        HM_TRY(h->rids.ensure(rn), "alloc");
+        HM_TRY(h->rdists.ensure(rn), "alloc");
+        HM_TRY(hipMemcpyAsync(h->q32.p, rerank_q, (size_t)nq * h->dim * 4, hipMemcpyHostToDevice,
+                              s),
+               "copy queries");
+        HM_TRY(rerank_device(h, h->q32.p, h->ids.p, nq, o->kstride, rerank_k, h->rids.p,
+                             h->rdists.p, s),
+               "rerank launch");
+        HM_TRY(hipMemcpyAsync(ids, h->rids.p, rn * 8, hipMemcpyDeviceToHost, s), "copy ids");
+        HM_TRY(hipMemcpyAsync(dists, h->rdists.p, rn * 8, hipMemcpyDeviceToHost, s), "copy");
+    } else {
+        HM_TRY(hipMemcpyAsync(ids, h->ids.p, on * 8, hipMemcpyDeviceToHost, s), "copy ids");
+        HM_TRY(hipMemcpyAsync(dists, h->dists.p, on * 8, hipMemcpyDeviceToHost, s), "copy dists");
+    }
     HM_TRY(hipMemcpyAsync(dcv.data(), h->dc.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
     HM_TRY(hipMemcpyAsync(hv.data(), h->hops.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
     HM_TRY(order_after(h, s), "stream order");
@@ -870,6 +889,20 @@ int hm_search_codes(hm_index *h, const double *queries, const hm_search_opts *o,
     if (!h || !queries || !o || !ids || !dists) return invalid("null argument");
     if (!h->has_codes) return invalid("no codes attached");
     return search_host(h, kSeqU8, nullptr, queries, o, ids, dists, dcount, hops);
+}
+
+int hm_search_codes_rerank(hm_index *h, const double *code_queries, const float *queries,
+                           const hm_search_opts *o, int32_t k, int64_t *ids, double *dists,
+                           int64_t *dcount, int64_t *hops) {
+    if (!h || !code_queries || !queries || !o || !ids || !dists) return invalid("null argument");
+    if (!h->has_codes) return invalid("no codes attached");
+    if (o->k || o->ef || o->mask_pool) return invalid("the quantized search takes uniform k and ef");
+    if (k < 1 || k > o->kstride || o->k_all != o->kstride)
+        return invalid("k must be in [1, fetch] and the traversal must keep fetch candidates");
+    int p2 = 1;
+    while (p2 < o->kstride) p2 <<= 1;
+    if (p2 * 12 + 16 + h->dim * 8 > 160 * 1024) return invalid("rerank fetch too large");
+    return search_host(h, kSeqU8, nullptr, code_queries, o, ids, dists, dcount, hops, queries, k);
 }
 
 int hm_set_tuning(hm_index *h, int32_t fast_hash_log2, int32_t hash_log2, int32_t waves_per_cu,

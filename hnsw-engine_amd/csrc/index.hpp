@@ -40,8 +40,8 @@ struct hm_index {
     hm::DevBuf<double> q64;
     hm::DevBuf<int32_t> kq, efq, midx, dc, hops, fail;
     hm::DevBuf<uint32_t> maskbits;
-    hm::DevBuf<int64_t> ids, cand;
-    hm::DevBuf<double> dists;
+    hm::DevBuf<int64_t> ids, cand, rids;
+    hm::DevBuf<double> dists, rdists;
     // learned stop (hm_set_stop_model)
     hm::DevBuf<float> stop_w;
     float stop_logit = 0.f;
@@ -76,7 +76,9 @@ struct hm_index {
         maskbits.release();
         ids.release();
         cand.release();
+        rids.release();
         dists.release();
+        rdists.release();
         stop_w.release();
         stats.release();
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);

@@ -72,6 +72,23 @@ __global__ __launch_bounds__(64) void rerank_kernel(const float *vec, int dim, i
     }
 }
 
+// Launches rerank_kernel over device candidates (nq, fetch) into device outputs (nq, k).
+hipError_t rerank_device(hm_index *h, const float *d_q, const int64_t *d_cand, int64_t nq,
+                         int fetch, int k, int64_t *d_ids, double *d_d, hipStream_t s) {
+    int p2 = 1;
+    while (p2 < fetch) p2 <<= 1;
+    const int lds = p2 * 12 + 16 + h->dim * 8;
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute((const void *)rerank_kernel,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(rerank_kernel, dim3((unsigned)nq), dim3(64), lds, s, h->vec.p, h->dim,
+                       h->metric, d_q, d_cand, fetch, k, p2, d_ids, d_d);
+    return hipGetLastError();
+}
+
 }  // namespace hm
 
 using namespace hm;
@@ -86,14 +103,9 @@ extern "C" int hm_rerank(hm_index *h, const float *queries, int64_t nq, const in
             return invalid("candidate ids outside [-1, n)");
     int p2 = 1;
     while (p2 < fetch) p2 <<= 1;
-    const int lds = p2 * 12 + 16 + h->dim * 8;
-    if (lds > 160 * 1024) return invalid("rerank fetch too large");
+    if (p2 * 12 + 16 + h->dim * 8 > 160 * 1024) return invalid("rerank fetch too large");
     hipStream_t s = h->stream;
     HM_TRY(hipSetDevice(h->device), "hipSetDevice");
-    if (lds > 64 * 1024)
-        HM_TRY(hipFuncSetAttribute((const void *)rerank_kernel,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds),
-               "attribute");
     HM_TRY(h->q32.ensure((size_t)nq * h->dim), "alloc");
     HM_TRY(h->cand.ensure((size_t)nq * std::max(fetch, 1)), "alloc");
     HM_TRY(h->ids.ensure((size_t)nq * k), "alloc");
@@ -104,9 +116,8 @@ extern "C" int hm_rerank(hm_index *h, const float *queries, int64_t nq, const in
         HM_TRY(hipMemcpyAsync(h->cand.p, candidates, (size_t)nq * fetch * 8, hipMemcpyHostToDevice,
                               s),
                "copy");
-    hipLaunchKernelGGL(rerank_kernel, dim3((unsigned)nq), dim3(64), lds, s, h->vec.p, h->dim,
-                       h->metric, h->q32.p, h->cand.p, fetch, k, p2, h->ids.p, h->dists.p);
-    HM_TRY(hipGetLastError(), "rerank launch");
+    HM_TRY(rerank_device(h, h->q32.p, h->cand.p, nq, fetch, k, h->ids.p, h->dists.p, s),
+           "rerank launch");
     HM_TRY(hipMemcpyAsync(ids, h->ids.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, s), "copy");
     HM_TRY(hipMemcpyAsync(dists, h->dists.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, s), "copy");
     HM_TRY(hipStreamSynchronize(s), "rerank");

@@ -8,13 +8,13 @@ insertion on the host, identical to the reference's.
 """
 
 from .exact import exact_distances, exact_search
-from .index import ENGINES, SELECTIONS, HNSWIndex, IndexStats
+from .graph_io import GraphState
+from .index import ENGINES, SELECTIONS, HNSWIndex, IndexParams
 from .metrics import METRICS
-from .persistence import GraphState
 from .quantization import QuantizedIndex, ScalarQuantizer
 from .recall import recall_at_k
 
 __all__ = [
-    "ENGINES", "METRICS", "SELECTIONS", "GraphState", "HNSWIndex", "IndexStats",
+    "ENGINES", "METRICS", "SELECTIONS", "GraphState", "HNSWIndex", "IndexParams",
     "QuantizedIndex", "ScalarQuantizer", "exact_distances", "exact_search", "recall_at_k",
 ]

@@ -17,6 +17,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libhnsw_mi355x.so"
 SYMBOLS = (
     "hm_open",
     "hm_close",
+    "hm_check_graph",
     "hm_has_exact_u8",
     "hm_search",
     "hm_search_dev",
@@ -25,6 +26,7 @@ SYMBOLS = (
     "hm_queries_exact",
     "hm_attach_codes",
     "hm_search_codes",
+    "hm_search_codes_rerank",
     "hm_rerank",
     "hm_exact",
     "hm_exact_dev",
@@ -104,6 +106,7 @@ def lib() -> ctypes.CDLL:
         sig = {
             "hm_open": ([ctypes.POINTER(NativeGraph), ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
             "hm_close": ([vp], None),
+            "hm_check_graph": ([ctypes.POINTER(NativeGraph)], ctypes.c_int),
             "hm_has_exact_u8": ([vp], ctypes.c_int),
             "hm_search": ([vp, vp, ctypes.POINTER(SearchOpts), vp, vp, vp, vp], ctypes.c_int),
             "hm_search_dev": (
@@ -121,6 +124,8 @@ def lib() -> ctypes.CDLL:
             "hm_queries_exact": ([vp, i64], ctypes.c_int),
             "hm_attach_codes": ([vp, vp], ctypes.c_int),
             "hm_search_codes": ([vp, vp, ctypes.POINTER(SearchOpts), vp, vp, vp, vp], ctypes.c_int),
+            "hm_search_codes_rerank": ([vp, vp, vp, ctypes.POINTER(SearchOpts), i32, vp, vp, vp,
+                                        vp], ctypes.c_int),
             "hm_rerank": ([vp, vp, i64, vp, i32, i32, vp, vp], ctypes.c_int),
             "hm_exact": ([vp, i64, i32, i32, vp, i64, i32, vp, ctypes.c_int, vp, vp], ctypes.c_int),
             "hm_exact_dev": ([vp, i64, i32, i32, vp, i64, i32, vp, ctypes.c_int, vp, vp, vp],

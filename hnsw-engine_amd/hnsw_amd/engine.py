@@ -18,7 +18,7 @@ from typing import Any
 import numpy as np
 
 from . import _native
-from .persistence import STORAGE, GraphState, native_from_csr, state_from_native
+from .graph_io import STORAGE, GraphState, native_from_csr, state_from_native
 
 _INITIAL_CAPACITY = 1024
 
@@ -171,19 +171,26 @@ class MI355XEngine:
                     _native.ptr(dists), _native.ptr(dc), _native.ptr(hops),
                 )
             )
-        self._counters[0] += int(dc.sum())
-        self._counters[1] += int(hops.sum())
+        self._count(dc, hops)
         return ids, dists, dc, hops
 
-    def batch_search_codes(self, codes: np.ndarray, queries: np.ndarray, k: int, ef: int):
-        """Algorithm 5 over uint8 ``codes`` with float64 code-space queries (no filter)."""
+    def _attach_codes(self, codes: np.ndarray) -> ctypes.c_void_p:
         if codes.shape != (self._size, self.dim) or codes.dtype != np.uint8:
             raise ValueError(f"codes must be uint8 with shape {(self._size, self.dim)}")
         handle = self._device()
-        if self._codes is None or self._codes is not codes:
+        if self._codes is not codes:
             c = np.ascontiguousarray(codes)
             _native.check(_native.lib().hm_attach_codes(handle, _native.ptr(c)))
             self._codes = codes
+        return handle
+
+    def _count(self, dc: np.ndarray, hops: np.ndarray) -> None:
+        self._counters[0] += int(dc.sum())
+        self._counters[1] += int(hops.sum())
+
+    def batch_search_codes(self, codes: np.ndarray, queries: np.ndarray, k: int, ef: int):
+        """Algorithm 5 over uint8 ``codes`` with float64 code-space queries (no filter)."""
+        handle = self._attach_codes(codes)
         q = np.ascontiguousarray(queries, dtype=np.float64)
         nq = q.shape[0]
         opts, keep, kmax = self._opts(nq, k, ef, None, None, None)
@@ -198,8 +205,27 @@ class MI355XEngine:
                     _native.ptr(dists), _native.ptr(dc), _native.ptr(hops),
                 )
             )
-        self._counters[0] += int(dc.sum())
-        self._counters[1] += int(hops.sum())
+        self._count(dc, hops)
+        return ids, dists
+
+    def quantized_search(self, codes: np.ndarray, code_queries: np.ndarray, queries: np.ndarray,
+                         k: int, fetch: int, ef: int):
+        """The codes traversal keeping ``fetch`` candidates, then their exact re-rank, in one
+        device call (``hm_search_codes_rerank``): the best ``k`` ids and float64 distances."""
+        handle = self._attach_codes(codes)
+        cq = np.ascontiguousarray(code_queries, dtype=np.float64)
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = q.shape[0]
+        opts, keep, _ = self._opts(nq, fetch, ef, None, None, None)
+        ids = np.empty((nq, k), dtype=np.int64)
+        dists = np.empty((nq, k), dtype=np.float64)
+        dc = np.zeros(nq, dtype=np.int64)
+        hops = np.zeros(nq, dtype=np.int64)
+        if nq:
+            _native.check(_native.lib().hm_search_codes_rerank(
+                handle, _native.ptr(cq), _native.ptr(q), ctypes.byref(opts), int(k),
+                _native.ptr(ids), _native.ptr(dists), _native.ptr(dc), _native.ptr(hops)))
+        self._count(dc, hops)
         return ids, dists
 
     def rerank(self, queries: np.ndarray, candidates: np.ndarray, k: int):

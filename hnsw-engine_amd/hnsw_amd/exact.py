@@ -11,7 +11,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import _native
-from ._validation import as_matrix, check_int, check_k
+from . import checks
 from .metrics import validate_metric
 
 
@@ -24,14 +24,14 @@ def exact_search(data, queries, k: int, metric: str = "l2", *, query_block: int 
     ``-1`` / ``inf``.
     """
     metric_name = validate_metric(metric)
-    x = as_matrix(data, "data")
+    x = checks.real_array(data, "data", 2)
     n, dim = x.shape
     if n == 0 or dim == 0:
         raise ValueError(f"data must be non-empty with at least one dimension, got {x.shape}")
-    q = as_matrix(queries, "queries", dim=dim)
-    k = check_k(k, n)
-    check_int(query_block, "query_block", minimum=1)
-    check_int(data_block, "data_block", minimum=1)
+    q = checks.real_array(queries, "queries", 2, dim)
+    k = checks.neighbour_count(k, n)
+    checks.integer(query_block, "query_block", minimum=1)
+    checks.integer(data_block, "data_block", minimum=1)
     if metric_name == "cosine":
         for arr, name in ((x, "data"), (q, "queries")):
             zero = np.flatnonzero(np.linalg.norm(np.asarray(arr, np.float64), axis=1) == 0.0)
@@ -73,8 +73,8 @@ def exact_distances(data, queries, ids, metric: str = "l2") -> np.ndarray:
     Scoring helper (``oracle.py:129-150``): rows are gathered and summed in float64.
     """
     metric_name = validate_metric(metric)
-    x = as_matrix(data, "data")
-    q = as_matrix(queries, "queries", dim=x.shape[1])
+    x = checks.real_array(data, "data", 2)
+    q = checks.real_array(queries, "queries", 2, x.shape[1])
     id_array = np.asarray(ids, dtype=np.int64)
     if id_array.ndim != 2 or id_array.shape[0] != q.shape[0]:
         raise ValueError(f"ids must have shape (n_queries, m), got {id_array.shape}")

@@ -1,431 +1,306 @@
-"""``HNSWIndex``: the reference's public index API (``index.py:96-466``) on the MI355X engine.
+"""``HNSWIndex``: the reference's index API (``index.py:96-466``) over the MI355X engine.
 
-Validation, metric preparation, the seeded level draw, the search argument rules
-(``ef = max(ef, k)``; an all-false filter returns padding without searching) and the native
-file format are the reference's; graph construction runs the host builder and every search runs
-the HIP kernels. ``engine="mi355x"`` is the only engine here.
+The product boundary is :class:`hnsw_amd.engine.MI355XEngine`, the reference's ``Engine``
+protocol; this facade gives it the reference's constructor, search and persistence surface so
+that a caller of ``hnsw_engine.HNSWIndex`` finds the same arguments, results, padding and
+errors. Construction is sequential insertion on the host (the same graphs as the reference);
+every search runs the HIP kernels.
+
+Search rules kept from the reference (``index.py:244-312``): ``1 <= k <= len(index)``; ``ef``
+defaults to ``ef_search`` and is raised to ``k``; a filter is a boolean array with one entry
+per indexed vector and applies to every query; a filter that allows nothing returns padding
+(-1 / inf) without searching. Extensions: per-query ``k`` / ``ef`` / filters in one batch
+(:meth:`batch_search_mixed`) and the filter planner (:meth:`batch_search_planned`).
 """
 
 from __future__ import annotations
 
 import math
-from dataclasses import asdict, dataclass
+from dataclasses import asdict, dataclass, replace
 from pathlib import Path
-from typing import Any
+from typing import Any, NamedTuple
 
 import numpy as np
 
-from ._validation import as_matrix, as_vector, check_int, check_k
+from . import checks
 from .engine import MI355XEngine
-from .metrics import prepare, validate_metric
-from .persistence import (
+from .graph_io import (
     GraphState,
-    check_native_shapes,
+    check_native_graph,
+    check_native_layout,
     load_native,
     load_state,
     save_native,
     save_state,
-    validate_native,
 )
+from .metrics import METRICS, prepare
 
 ENGINES = ("mi355x",)
+# Engine names a file written by the reference may carry; they open on this engine.
+FOREIGN_ENGINES = ("numba", "reference")
 SELECTIONS = ("simple", "heuristic")
 
 
 @dataclass(frozen=True)
-class IndexStats:
-    """Sizes, layer structure, degrees, compact memory estimate and cumulative counters
-    (``index.py:63-93``)."""
+class IndexParams:
+    """Construction parameters, checked in the reference's order and with its messages."""
 
-    size: int
     dim: int
-    metric: str
-    engine: str
-    M: int
-    max_links_layer0: int
-    ef_construction: int
-    ef_search: int
-    m_L: float
-    selection: str
-    extend_candidates: bool
-    keep_pruned_connections: bool
-    max_level: int
-    entry_point: int
-    nodes_per_layer: tuple[int, ...]
-    links_per_layer: tuple[int, ...]
-    mean_degree_per_layer: tuple[float, ...]
-    memory_bytes: int
-    distance_computations: int
-    hops: int
+    metric: str = "l2"
+    M: int = 16
+    ef_construction: int = 200
+    ef_search: int = 50
+    seed: int | None = None
+    selection: str = "heuristic"
+    extend_candidates: bool = False
+    keep_pruned_connections: bool = False
+    m_L: float | None = None
+    engine: str = "mi355x"
 
-    def as_dict(self) -> dict[str, Any]:
-        return asdict(self)
+    def __post_init__(self) -> None:
+        fix = lambda field, value: object.__setattr__(self, field, value)  # noqa: E731
+        fix("dim", checks.integer(self.dim, "dim", minimum=1))
+        checks.one_of(self.metric, METRICS, "metric")
+        fix("M", checks.integer(self.M, "M", minimum=2))
+        fix("ef_construction", checks.integer(self.ef_construction, "ef_construction", minimum=1))
+        fix("ef_search", checks.integer(self.ef_search, "ef_search", minimum=1))
+        if self.seed is not None:
+            fix("seed", checks.integer(self.seed, "seed", minimum=0))
+        checks.one_of(self.selection, SELECTIONS, "selection")
+        checks.flag(self.extend_candidates, "extend_candidates")
+        checks.flag(self.keep_pruned_connections, "keep_pruned_connections")
+        if self.selection == "simple" and (self.extend_candidates or self.keep_pruned_connections):
+            raise ValueError(
+                "extend_candidates and keep_pruned_connections apply to selection='heuristic'")
+        fix("m_L", 1.0 / math.log(self.M) if self.m_L is None else checks.level_factor(self.m_L))
+        checks.one_of(self.engine, ENGINES, "engine")
 
 
-def sample_levels(rng: np.random.Generator, n: int, m_l: float) -> np.ndarray:
-    """``floor(-ln(U) * m_L)`` with ``U = 1 - rng.random()`` (``levels.py:22-31``)."""
-    u = 1.0 - rng.random(n)
-    return np.floor(-np.log(u) * m_l).astype(np.int64)
+class _Request(NamedTuple):
+    k: int
+    ef: int
+    mask: np.ndarray | None
+    nothing_allowed: bool
 
 
 class HNSWIndex:
-    """Hierarchical Navigable Small World index searched on an MI355X.
+    """Hierarchical Navigable Small World index whose searches run on an MI355X.
 
-    Arguments as in the reference: ``dim``, ``metric`` (``"l2"``, ``"cosine"``, ``"ip"``),
-    ``M``, ``ef_construction``, ``ef_search``, ``seed``, ``selection``,
-    ``extend_candidates``, ``keep_pruned_connections``, ``m_L``, ``engine`` (``"mi355x"``),
-    plus ``device`` (HIP device ordinal).
+    Arguments are the reference's (``dim``, ``metric``, ``M``, ``ef_construction``,
+    ``ef_search``, ``seed``, ``selection``, ``extend_candidates``, ``keep_pruned_connections``,
+    ``m_L``, ``engine``), with ``engine="mi355x"``, plus ``device``, the HIP device ordinal.
     """
 
-    def __init__(
-        self,
-        dim: int,
-        metric: str = "l2",
-        M: int = 16,
-        ef_construction: int = 200,
-        ef_search: int = 50,
-        seed: int | None = None,
-        selection: str = "heuristic",
-        extend_candidates: bool = False,
-        keep_pruned_connections: bool = False,
-        m_L: float | None = None,
-        engine: str = "mi355x",
-        device: int = 0,
-    ) -> None:
-        self._dim = check_int(dim, "dim", minimum=1)
-        self._metric = validate_metric(metric)
-        self._M = check_int(M, "M", minimum=2)
-        self._ef_construction = check_int(ef_construction, "ef_construction", minimum=1)
-        self._ef_search = check_int(ef_search, "ef_search", minimum=1)
-        self._seed = None if seed is None else check_int(seed, "seed", minimum=0)
-        if selection not in SELECTIONS:
-            raise ValueError(f"unknown selection {selection!r}; expected one of {SELECTIONS}")
-        self._selection = selection
-        self._extend = _check_bool(extend_candidates, "extend_candidates")
-        self._keep_pruned = _check_bool(keep_pruned_connections, "keep_pruned_connections")
-        if selection == "simple" and (self._extend or self._keep_pruned):
-            raise ValueError(
-                "extend_candidates and keep_pruned_connections apply to selection='heuristic'"
-            )
-        self._m_L = 1.0 / math.log(self._M) if m_L is None else _check_m_l(m_L)
-        if engine not in ENGINES:
-            raise ValueError(f"unknown engine {engine!r}; expected one of {ENGINES}")
-        self._engine_name = engine
-        self._rng = np.random.default_rng(self._seed)
-        self._engine = MI355XEngine(
-            self._dim, self._metric, self._M, self._ef_construction, self._selection,
-            self._extend, self._keep_pruned, device=device,
-        )
+    def __init__(self, dim: int, *args, device: int = 0, **kwargs) -> None:
+        self._params = IndexParams(dim, *args, **kwargs)
+        self._rng = np.random.default_rng(self._params.seed)
+        p = self._params
+        self._engine = MI355XEngine(p.dim, p.metric, p.M, p.ef_construction, p.selection,
+                                    p.extend_candidates, p.keep_pruned_connections, device=device)
 
-    # -- properties ---------------------------------------------------------------------
+    # -- parameters and state ---------------------------------------------------------------
 
-    @property
-    def dim(self) -> int:
-        return self._dim
-
-    @property
-    def metric(self) -> str:
-        return self._metric
-
-    @property
-    def M(self) -> int:
-        return self._M
-
-    @property
-    def ef_construction(self) -> int:
-        return self._ef_construction
+    dim = property(lambda self: self._params.dim)
+    metric = property(lambda self: self._params.metric)
+    M = property(lambda self: self._params.M)
+    ef_construction = property(lambda self: self._params.ef_construction)
+    m_L = property(lambda self: self._params.m_L)
+    selection = property(lambda self: self._params.selection)
+    engine = property(lambda self: self._params.engine)
+    entry_point = property(lambda self: self._engine.entry_point)
+    max_level = property(lambda self: self._engine.max_level)
+    distance_computations = property(lambda self: self._engine.distance_computations)
+    hops = property(lambda self: self._engine.hops)
 
     @property
     def ef_search(self) -> int:
-        return self._ef_search
+        return self._params.ef_search
 
     @ef_search.setter
     def ef_search(self, value: int) -> None:
-        self._ef_search = check_int(value, "ef_search", minimum=1)
-
-    @property
-    def m_L(self) -> float:
-        return self._m_L
-
-    @property
-    def selection(self) -> str:
-        return self._selection
-
-    @property
-    def engine(self) -> str:
-        return self._engine_name
-
-    @property
-    def entry_point(self) -> int:
-        return self._engine.entry_point
-
-    @property
-    def max_level(self) -> int:
-        return self._engine.max_level
+        self._params = replace(self._params, ef_search=value)
 
     def __len__(self) -> int:
         return self._engine.size
 
     def __repr__(self) -> str:
-        return (
-            f"HNSWIndex(dim={self._dim}, metric={self._metric!r}, M={self._M}, "
-            f"ef_construction={self._ef_construction}, ef_search={self._ef_search}, "
-            f"selection={self._selection!r}, engine={self._engine_name!r}, size={len(self)})"
-        )
+        p = self._params
+        return (f"HNSWIndex(dim={p.dim}, metric={p.metric!r}, M={p.M}, "
+                f"ef_construction={p.ef_construction}, ef_search={p.ef_search}, "
+                f"selection={p.selection!r}, engine={p.engine!r}, size={len(self)})")
 
-    # -- building -------------------------------------------------------------------------
-
-    def add(self, vectors) -> np.ndarray:
-        """Insert ``(n, dim)`` vectors (sequential insertion on the host); returns their ids."""
-        prepared = self._prepare(vectors)
-        levels = sample_levels(self._rng, prepared.shape[0], self._m_L)
-        return self._insert(prepared, levels)
-
-    def _add_with_levels(self, vectors, levels) -> np.ndarray:
-        prepared = self._prepare(vectors)
-        level_array = np.asarray(levels, dtype=np.int64)
-        if level_array.shape != (prepared.shape[0],) or (level_array < 0).any():
-            raise ValueError("levels must be a non-negative integer array, one per vector")
-        return self._insert(prepared, level_array)
-
-    def _insert(self, prepared: np.ndarray, levels: np.ndarray) -> np.ndarray:
-        start = len(self)
-        if prepared.shape[0]:
-            self._engine.insert(prepared, levels)
-        return np.arange(start, start + prepared.shape[0], dtype=np.int64)
-
-    def _prepare(self, vectors) -> np.ndarray:
-        return prepare(as_matrix(vectors, "vectors", dim=self._dim), self._metric)
-
-    # -- search ---------------------------------------------------------------------------
-
-    def search(self, query, k: int = 10, ef: int | None = None, filter=None):
-        """Approximate ``k`` nearest neighbours of one query; ``(ids, distances)`` of length
-        ``k``, padded with ``-1`` / ``inf``."""
-        k_valid, ef_valid = self._check_search_args(k, ef)
-        allowed = self._check_filter(filter)
-        prepared = self._prepare(as_vector(query, "query", dim=self._dim)[None, :])[0]
-        if allowed is not None and not allowed.any():
-            return np.full(k_valid, -1, dtype=np.int64), np.full(k_valid, np.inf)
-        return self._engine.search(prepared, k_valid, ef_valid, allowed)
-
-    def batch_search(self, queries, k: int = 10, ef: int | None = None, filter=None):
-        """:meth:`search` for every row of ``queries``; one filter for every query."""
-        k_valid, ef_valid = self._check_search_args(k, ef)
-        allowed = self._check_filter(filter)
-        prepared = self._prepare(queries)
-        if allowed is not None and not allowed.any():
-            shape = (prepared.shape[0], k_valid)
-            return np.full(shape, -1, dtype=np.int64), np.full(shape, np.inf)
-        return self._engine.batch_search(prepared, k_valid, ef_valid, allowed)
-
-    def batch_search_planned(self, queries, k: int = 10, ef: int | None = None, filter=None,
-                             planner=None):
-        """Extension: :meth:`batch_search` with a filter planner (``planner.FilterPlanner``,
-        default thresholds when None). Returns ``(ids, dists, plan)``: the in-graph filtered
-        search (the reference's results) when ``plan.path == "graph"``, the exact top-k over
-        the allowed vectors when ``"exact"``. Without a filter this is :meth:`batch_search`."""
-        from .planner import FilterPlan, FilterPlanner
-
-        k_valid, ef_valid = self._check_search_args(k, ef)
-        allowed = self._check_filter(filter)
-        prepared = self._prepare(queries)
-        shape = (prepared.shape[0], k_valid)
-        if allowed is None:
-            ids, dists = self._engine.batch_search(prepared, k_valid, ef_valid, None)
-            return ids, dists, FilterPlan("graph", 1.0, 1.0, 0.0)
-        if not allowed.any():
-            return (np.full(shape, -1, dtype=np.int64), np.full(shape, np.inf),
-                    FilterPlan("exact", 0.0, 1.0, 0.0))
-        plan = (planner or FilterPlanner()).plan(self._engine, allowed)
-        if plan.path == "graph":
-            ids, dists = self._engine.batch_search(prepared, k_valid, ef_valid, allowed)
-            return ids, dists, plan
-        kk = min(k_valid, int(allowed.sum()))
-        got_ids, got_d = self._engine.exact_search(prepared, kk, allowed)
-        ids = np.full(shape, -1, dtype=np.int64)
-        dists = np.full(shape, np.inf)
-        ids[:, :kk] = got_ids
-        dists[:, :kk] = got_d
-        return ids, dists, plan
-
-    def batch_search_mixed(self, queries, k, ef, filters=None, filter_index=None):
-        """Extension: one batch with per-query ``k`` and ``ef`` (arrays) and per-query filters
-        (``filters``: a pool of boolean rows; ``filter_index[i]`` picks query i's row, -1 for
-        none). Each query follows exactly the reference semantics of its own arguments.
-        Returns ``(ids, dists)`` of shape ``(nq, max k)``; a query's row is padded past its k."""
-        if len(self) == 0:
-            raise ValueError("cannot search an empty index")
-        prepared = self._prepare(queries)
-        nq = prepared.shape[0]
-        k_arr = np.broadcast_to(np.asarray(k, dtype=np.int64), (nq,))
-        ef_arr = np.broadcast_to(np.asarray(ef, dtype=np.int64), (nq,))
-        if nq and (k_arr.min() < 1 or k_arr.max() > len(self)):
-            raise ValueError("every k must be in [1, len(index)]")
-        if nq and ef_arr.min() < 1:
-            raise ValueError("every ef must be >= 1")
-        pool = None
-        index = None
-        if filters is not None:
-            pool = np.asarray(filters)
-            if pool.dtype != np.bool_:
-                raise TypeError(f"filter must be a boolean array, got dtype {pool.dtype}")
-            pool = np.atleast_2d(pool)
-            if pool.shape[1] != len(self):
-                raise ValueError("every filter must have one entry per indexed vector")
-            index = np.broadcast_to(np.asarray(filter_index, dtype=np.int64), (nq,)).copy()
-            if (index >= pool.shape[0]).any():
-                raise ValueError("filter_index out of range")
-            empty = ~pool.any(axis=1)
-            index[(index >= 0) & empty[np.maximum(index, 0)]] = -2  # all-false: padding only
-        ids = np.full((nq, int(k_arr.max()) if nq else 1), -1, dtype=np.int64)
-        dists = np.full(ids.shape, np.inf)
-        run = np.ones(nq, dtype=bool) if index is None else index != -2
-        if run.any():
-            sel = np.flatnonzero(run)
-            got_ids, got_d, _, _ = self._engine.search_many(
-                prepared[sel], k_arr[sel], np.maximum(ef_arr[sel], k_arr[sel]),
-                mask_pool=pool, mask_index=None if index is None else index[sel],
-                kstride=ids.shape[1],
-            )
-            ids[sel] = got_ids
-            dists[sel] = got_d
-        return ids, dists
-
-    def _check_search_args(self, k: int, ef: int | None) -> tuple[int, int]:
-        if len(self) == 0:
-            raise ValueError("cannot search an empty index")
-        k_valid = check_k(k, len(self))
-        ef_valid = self._ef_search if ef is None else check_int(ef, "ef", minimum=1)
-        return k_valid, max(ef_valid, k_valid)
-
-    def _check_filter(self, filter) -> np.ndarray | None:
-        if filter is None:
-            return None
-        mask = np.asarray(filter)
-        if mask.dtype != np.bool_:
-            raise TypeError(f"filter must be a boolean array, got dtype {mask.dtype}")
-        if mask.shape != (len(self),):
-            raise ValueError(
-                f"filter must have one entry per indexed vector ({len(self)}), got shape "
-                f"{mask.shape}"
-            )
-        return np.ascontiguousarray(mask)
-
-    # -- inspection -----------------------------------------------------------------------
+    def reset_counters(self) -> None:
+        """Zero the cumulative distance-computation and hop counters."""
+        self._engine.distance_computations = 0
+        self._engine.hops = 0
 
     def graph_state(self) -> GraphState:
         return self._engine.export_state()
 
-    def neighbors(self, node: int, layer: int = 0) -> list[int]:
-        state = self.graph_state()
-        node = check_int(node, "node", minimum=0)
-        layer = check_int(layer, "layer", minimum=0)
-        if node >= state.size or layer > int(state.levels[node]):
-            raise ValueError(f"node {node} is not present on layer {layer}")
-        return state.adjacency(layer)[node]
+    # -- building ------------------------------------------------------------------------------
 
-    def stats(self) -> IndexStats:
-        state = self.graph_state()
-        nodes = tuple(int((state.levels >= layer).sum()) for layer in range(state.num_layers))
-        links = tuple(int(state.indices[layer].shape[0]) for layer in range(state.num_layers))
-        degrees = tuple(lk / nd if nd else 0.0 for lk, nd in zip(links, nodes))
-        link_slots = sum(
-            nd * ((2 * self._M if layer == 0 else self._M) + 1) for layer, nd in enumerate(nodes)
-        )
-        memory = state.size * self._dim * 4 + link_slots * 4 + state.size * 4
-        return IndexStats(
-            size=state.size, dim=self._dim, metric=self._metric, engine=self._engine_name,
-            M=self._M, max_links_layer0=2 * self._M, ef_construction=self._ef_construction,
-            ef_search=self._ef_search, m_L=self._m_L, selection=self._selection,
-            extend_candidates=self._extend, keep_pruned_connections=self._keep_pruned,
-            max_level=state.max_level, entry_point=state.entry_point, nodes_per_layer=nodes,
-            links_per_layer=links, mean_degree_per_layer=degrees, memory_bytes=memory,
-            distance_computations=self._engine.distance_computations, hops=self._engine.hops,
-        )
+    def add(self, vectors) -> np.ndarray:
+        """Insert ``(n, dim)`` vectors by sequential insertion on the host; returns their ids.
+        Levels are ``floor(-ln(1 - U) * m_L)`` from the index's generator (``levels.py:22-31``)."""
+        rows = self._prepare(vectors)
+        u = 1.0 - self._rng.random(rows.shape[0])
+        return self._insert(rows, np.floor(-np.log(u) * self._params.m_L).astype(np.int64))
 
-    @property
-    def distance_computations(self) -> int:
-        return self._engine.distance_computations
+    def _add_with_levels(self, vectors, levels) -> np.ndarray:
+        rows = self._prepare(vectors)
+        lv = np.asarray(levels, dtype=np.int64)
+        if lv.shape != (rows.shape[0],) or (lv < 0).any():
+            raise ValueError("levels must be a non-negative integer array, one per vector")
+        return self._insert(rows, lv)
 
-    @property
-    def hops(self) -> int:
-        return self._engine.hops
+    def _insert(self, rows: np.ndarray, levels: np.ndarray) -> np.ndarray:
+        first = len(self)
+        if rows.shape[0]:
+            self._engine.insert(rows, levels)
+        return np.arange(first, first + rows.shape[0], dtype=np.int64)
 
-    def reset_counters(self) -> None:
-        self._engine.distance_computations = 0
-        self._engine.hops = 0
+    def _prepare(self, vectors) -> np.ndarray:
+        return prepare(checks.real_array(vectors, "vectors", 2, self.dim), self.metric)
 
-    # -- persistence (the reference's native format) ---------------------------------------
+    # -- search -------------------------------------------------------------------------------
 
-    def save(self, path: str | Path) -> None:
-        """The index (parameters, generator state, graph) as a ``.npz`` snapshot at ``path``
-        (the reference's ``HNSWIndex.save``, ``index.py:369-372``; same file format)."""
-        meta = {"params": self._params(), "rng_state": self._rng.bit_generator.state}
-        save_state(path, meta, self.graph_state())
+    def _request(self, k, ef, filter) -> _Request:
+        if len(self) == 0:
+            raise ValueError("cannot search an empty index")
+        kk = checks.neighbour_count(k, len(self))
+        width = self.ef_search if ef is None else checks.integer(ef, "ef", minimum=1)
+        mask = checks.node_mask(filter, len(self))
+        return _Request(kk, max(width, kk), mask, mask is not None and not mask.any())
+
+    def _run(self, req: _Request, rows: np.ndarray):
+        if req.nothing_allowed:
+            shape = (rows.shape[0], req.k)
+            return np.full(shape, -1, dtype=np.int64), np.full(shape, np.inf)
+        return self._engine.batch_search(rows, req.k, req.ef, req.mask)
+
+    def search(self, query, k: int = 10, ef: int | None = None, filter=None):
+        """Approximate ``k`` nearest neighbours of one ``(dim,)`` query: ``(ids, distances)``
+        of length ``k``, closest first, padded with -1 / inf."""
+        req = self._request(k, ef, filter)
+        row = self._prepare(checks.real_array(query, "query", 1, self.dim)[None, :])
+        ids, dists = self._run(req, row)
+        return ids[0], dists[0]
+
+    def batch_search(self, queries, k: int = 10, ef: int | None = None, filter=None):
+        """:meth:`search` for every row of ``(nq, dim)`` queries; ``(nq, k)`` results."""
+        req = self._request(k, ef, filter)
+        return self._run(req, self._prepare(queries))
+
+    def batch_search_planned(self, queries, k: int = 10, ef: int | None = None, filter=None,
+                             planner=None):
+        """Extension: :meth:`batch_search` through a filter planner (``planner.FilterPlanner``,
+        default thresholds when None). Returns ``(ids, dists, plan)``: the in-graph filtered
+        search (the reference's results) when ``plan.path == "graph"``, the exact top-k over
+        the allowed vectors when ``"exact"``."""
+        from .planner import FilterPlan, FilterPlanner
+
+        req = self._request(k, ef, filter)
+        rows = self._prepare(queries)
+        if req.mask is None:
+            return (*self._run(req, rows), FilterPlan("graph", 1.0, 1.0, 0.0))
+        if req.nothing_allowed:
+            return (*self._run(req, rows), FilterPlan("exact", 0.0, 1.0, 0.0))
+        plan = (planner or FilterPlanner()).plan(self._engine, req.mask)
+        if plan.path == "graph":
+            return (*self._run(req, rows), plan)
+        got = min(req.k, int(req.mask.sum()))
+        ids = np.full((rows.shape[0], req.k), -1, dtype=np.int64)
+        dists = np.full(ids.shape, np.inf)
+        ids[:, :got], dists[:, :got] = self._engine.exact_search(rows, got, req.mask)
+        return ids, dists, plan
+
+    def batch_search_mixed(self, queries, k, ef, filters=None, filter_index=None):
+        """Extension: one batch with per-query ``k`` and ``ef`` (ints or arrays) and per-query
+        filters (``filters``: a pool of boolean rows; ``filter_index[i]`` picks query i's row,
+        -1 for none). Each query follows the reference's rules for its own arguments. Returns
+        ``(ids, dists)`` of shape ``(nq, max k)``; a row is padded past its query's k."""
+        if len(self) == 0:
+            raise ValueError("cannot search an empty index")
+        rows = self._prepare(queries)
+        nq = rows.shape[0]
+        kq = np.broadcast_to(np.asarray(k, dtype=np.int64), (nq,))
+        efq = np.broadcast_to(np.asarray(ef, dtype=np.int64), (nq,))
+        if nq and not (kq.min() >= 1 and kq.max() <= len(self)):
+            raise ValueError("every k must be in [1, len(index)]")
+        if nq and efq.min() < 1:
+            raise ValueError("every ef must be >= 1")
+        run = np.ones(nq, dtype=bool)
+        pool = which = None
+        if filters is not None:
+            pool = np.atleast_2d(np.asarray(filters))
+            if pool.dtype != np.bool_:
+                raise TypeError(f"filter must be a boolean array, got dtype {pool.dtype}")
+            if pool.shape[1] != len(self):
+                raise ValueError("every filter must have one entry per indexed vector")
+            which = np.broadcast_to(np.asarray(filter_index, dtype=np.int64), (nq,))
+            if (which >= pool.shape[0]).any():
+                raise ValueError("filter_index out of range")
+            # a query whose filter allows nothing gets padding and no search
+            run = (which < 0) | pool.any(axis=1)[np.maximum(which, 0)]
+        width = int(kq.max()) if nq else 1
+        ids = np.full((nq, width), -1, dtype=np.int64)
+        dists = np.full(ids.shape, np.inf)
+        sel = np.flatnonzero(run)
+        if sel.size:
+            got = self._engine.search_many(
+                rows[sel], kq[sel], np.maximum(efq[sel], kq[sel]), mask_pool=pool,
+                mask_index=None if which is None else which[sel], kstride=width)
+            ids[sel], dists[sel] = got[0], got[1]
+        return ids, dists
+
+    # -- persistence (the reference's formats) ----------------------------------------------------
+
+    def _meta(self) -> dict[str, Any]:
+        return {"params": asdict(self._params), "rng_state": self._rng.bit_generator.state}
+
+    def save(self, path) -> None:
+        """The parameters, generator state and graph as a ``.npz`` snapshot at ``path`` (the
+        reference's ``HNSWIndex.save`` format, ``index.py:369-372``)."""
+        save_state(path, self._meta(), self.graph_state())
 
     @classmethod
-    def load(cls, path: str | Path, engine: str | None = None) -> "HNSWIndex":
-        """An index written by :meth:`save` (or by the reference), validated; ``engine``
-        defaults to ``"mi355x"`` (the reference's engine names are accepted and mapped to it)."""
-        meta, state = load_state(path)
+    def _reopen(cls, meta: dict[str, Any], device: int = 0, engine: str | None = None):
         params = dict(meta["params"])
-        params["engine"] = "mi355x" if engine is None else engine
-        index = cls(**params)
+        named = params.get("engine") if engine is None else engine
+        params["engine"] = "mi355x" if named in FOREIGN_ENGINES else named
+        index = cls(device=device, **params)
+        index._rng.bit_generator.state = meta["rng_state"]
+        return index
+
+    @classmethod
+    def load(cls, path, engine: str | None = None, device: int = 0) -> "HNSWIndex":
+        """An index saved by :meth:`save` here or by the reference (its engine names open on
+        this engine)."""
+        meta, state = load_state(path)
+        index = cls._reopen(meta, device, engine)
         if state.vectors.shape[1:] != (index.dim,):
             raise ValueError(
-                f"stored vectors have shape {state.vectors.shape}, expected dim {index.dim}"
-            )
-        index._rng.bit_generator.state = meta["rng_state"]
+                f"stored vectors have shape {state.vectors.shape}, expected dim {index.dim}")
         index._engine.load_state(state)
         return index
 
-    def save_native(self, directory: str | Path) -> None:
-        """Write the native arrays and ``header.json`` (``index.py:390-403``)."""
-        meta = {
-            "params": self._params(),
-            "rng_state": self._rng.bit_generator.state,
-            "entry_point": self._engine.entry_point,
-            "max_level": self._engine.max_level,
-        }
+    def save_native(self, directory) -> None:
+        """The engine's arrays as ``.npy`` files plus ``header.json`` (``index.py:390-403``)."""
+        meta = {**self._meta(), "entry_point": self.entry_point, "max_level": self.max_level}
         save_native(directory, meta, self._engine.native_arrays())
 
     @classmethod
-    def load_native(cls, directory: str | Path, mmap: bool = True, validate: bool = True,
+    def load_native(cls, directory, mmap: bool = True, validate: bool = True,
                     device: int = 0) -> "HNSWIndex":
-        """Open a directory written by the reference's (or this) ``save_native``
-        (``index.py:405-425``); ``validate`` checks every link before the kernels use it."""
+        """Open a directory written by ``save_native`` here or by the reference
+        (``index.py:405-425``). With ``mmap`` the arrays are read-only maps used in place
+        (adding vectors copies them first); ``validate`` checks every link before any search
+        reads it (skip it only for files you wrote yourself)."""
         meta, arrays = load_native(directory, mmap=mmap)
-        params = {**meta["params"], "engine": "mi355x"}
-        index = cls(**params, device=device)
-        index._rng.bit_generator.state = meta["rng_state"]
+        index = cls._reopen(meta, device)
         entry, top = int(meta["entry_point"]), int(meta["max_level"])
-        check_native_shapes(arrays, index.dim, index.M)
+        check_native_layout(arrays, index.dim, index.M)
         if validate:
-            validate_native(arrays, entry, top, index.M)
+            check_native_graph(arrays, entry, top, index.M, index.metric)
         index._engine.attach_native(arrays, entry, top)
         return index
-
-    def _params(self) -> dict[str, Any]:
-        return {
-            "dim": self._dim, "metric": self._metric, "M": self._M,
-            "ef_construction": self._ef_construction, "ef_search": self._ef_search,
-            "seed": self._seed, "selection": self._selection,
-            "extend_candidates": self._extend, "keep_pruned_connections": self._keep_pruned,
-            "m_L": self._m_L, "engine": self._engine_name,
-        }
-
-
-def _check_bool(value: object, name: str) -> bool:
-    if not isinstance(value, bool):
-        raise TypeError(f"{name} must be a bool, got {type(value).__name__}")
-    return value
-
-
-def _check_m_l(value: object) -> float:
-    if isinstance(value, bool) or not isinstance(value, (int, float)):
-        raise TypeError(f"m_L must be a number, got {type(value).__name__}")
-    result = float(value)
-    if not math.isfinite(result) or result < 0:
-        raise ValueError(f"m_L must be finite and >= 0, got {value}")
-    return result

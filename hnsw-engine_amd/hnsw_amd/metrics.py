@@ -12,9 +12,9 @@ METRICS = ("l2", "cosine", "ip")
 
 
 def validate_metric(metric: object) -> str:
-    if metric not in METRICS:
-        raise ValueError(f"unknown metric {metric!r}; expected one of {METRICS}")
-    return metric  # type: ignore[return-value]
+    from .checks import one_of
+
+    return one_of(metric, METRICS, "metric")
 
 
 def prepare(vectors: np.ndarray, metric: str) -> np.ndarray:

@@ -1,12 +1,18 @@
-"""Scalar quantization and the quantized search (``quantization.py:29-138`` of the reference).
+"""8-bit scalar quantization and the quantized search (the reference's ``ScalarQuantizer`` and
+``QuantizedIndex``, ``quantization.py:29-138``).
 
-``ScalarQuantizer``: per-dimension offset (the minimum) and ONE shared step (the widest range
-over 255; 1 when every value is an integer and every range spans at most 256 values, i.e.
-lossless 8-bit data). Codes are ``clip(rint((x - offset) / step), 0, 255)`` as uint8.
+Codes: one byte per dimension, ``clip(rint((x - offset) / step), 0, 255)`` with ``offset`` the
+per-dimension minimum and ONE step shared by all dimensions (the widest range over 255, or 1
+when the data are integers spanning at most 256 values, which makes 8-bit data lossless). With a
+shared step the code-space distance is an increasing affine function of the distance to the
+decoded vectors, so a traversal over the codes ranks candidates as the decoded vectors would:
+for l2 the code-space query is ``(q - offset) / step`` and the distance scales by ``step**2``;
+for ip and cosine the query is unchanged and the distance maps through
+``step * d_code + bias(q)``.
 
-``QuantizedIndex.batch_search`` traverses the index's graph with distances to the codes
-(float64 code-space queries, the HIP codes kernel) and optionally re-ranks the best ``R`` with
-exact float64 distances to the float32 vectors (the HIP re-rank kernel).
+``QuantizedIndex.batch_search`` runs on the device in one call: the traversal over the codes
+keeps its ``fetch`` candidates in HBM and, with ``rerank > 0``, the exact float64 re-rank scores
+them there (``hm_search_codes_rerank``); only the final ``(nq, k)`` results cross PCIe.
 """
 
 from __future__ import annotations
@@ -15,62 +21,59 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._validation import as_matrix, check_int
+from . import checks
 
-CODE_LEVELS = 256
+LEVELS = 256
 
 
 @dataclass(frozen=True)
 class ScalarQuantizer:
-    offset: np.ndarray
+    offset: np.ndarray  # float64 per-dimension minimum
     step: float
 
     @classmethod
     def fit(cls, vectors) -> "ScalarQuantizer":
-        x = np.asarray(as_matrix(vectors, "vectors"), dtype=np.float64)
+        x = np.asarray(checks.real_array(vectors, "vectors", 2), dtype=np.float64)
         if x.shape[0] == 0:
             raise ValueError("cannot fit a quantizer on zero vectors")
         low = x.min(axis=0)
-        widest = float((x.max(axis=0) - low).max())
-        integer_valued = bool(np.array_equal(x, np.rint(x)))
-        if (integer_valued and widest <= CODE_LEVELS - 1) or widest == 0.0:
-            step = 1.0
-        else:
-            step = widest / (CODE_LEVELS - 1)
-        return cls(low, step)
+        span = float(np.max(x.max(axis=0) - low))
+        lossless = span == 0.0 or (span <= LEVELS - 1 and bool(np.all(np.rint(x) == x)))
+        return cls(low, 1.0 if lossless else span / (LEVELS - 1))
 
     def encode(self, vectors) -> np.ndarray:
-        x = np.asarray(vectors, dtype=np.float64)
-        codes = np.rint((x - self.offset) / self.step)
-        return np.ascontiguousarray(np.clip(codes, 0, CODE_LEVELS - 1), dtype=np.uint8)
+        """uint8 codes; values outside the fitted range clip to 0 / 255."""
+        scaled = (np.asarray(vectors, dtype=np.float64) - self.offset) / self.step
+        return np.ascontiguousarray(np.clip(np.rint(scaled), 0, LEVELS - 1).astype(np.uint8))
 
     def decode(self, codes: np.ndarray) -> np.ndarray:
         return self.offset + self.step * codes.astype(np.float64)
 
     def transform_queries(self, queries, metric: str) -> np.ndarray:
+        """float64 code-space queries: shifted and scaled for l2, unchanged otherwise."""
         q = np.asarray(queries, dtype=np.float64)
-        if metric == "l2":
-            return np.ascontiguousarray((q - self.offset) / self.step)
-        return np.ascontiguousarray(q)
+        return np.ascontiguousarray((q - self.offset) / self.step if metric == "l2" else q)
 
-    def to_distance(self, code_distances: np.ndarray, queries, metric: str) -> np.ndarray:
+    def to_distance(self, code_distances, queries, metric: str) -> np.ndarray:
+        """Code-space distances (one row per query) as distances to the decoded vectors."""
+        d = np.asarray(code_distances, dtype=np.float64)
         if metric == "l2":
-            return code_distances * self.step**2
-        shift = np.asarray(queries, dtype=np.float64) @ self.offset
-        if metric == "ip":
-            return self.step * code_distances - shift[:, None]
-        return 1.0 - shift[:, None] - self.step + self.step * code_distances
+            return d * self.step**2
+        # ip: -(offset.q) - step c.q; cosine: 1 - offset.q - step c.q, with d = -c.q / 1 - c.q
+        shift = (np.asarray(queries, dtype=np.float64) @ self.offset)[:, None]
+        bias = -shift if metric == "ip" else (1.0 - shift) - self.step
+        return self.step * d + bias
 
 
 class QuantizedIndex:
-    """Searches an :class:`HNSWIndex`'s graph with distances to uint8 codes."""
+    """Searches an :class:`HNSWIndex`'s graph (built from its float32 vectors) with distances
+    to 8-bit codes; the codes are a snapshot of the index at construction."""
 
     def __init__(self, index) -> None:
         if len(index) == 0:
             raise ValueError("cannot quantize an empty index")
         self._index = index
-        self._engine = index._engine
-        vectors = self._engine.native_arrays()["vectors"]
+        vectors = index._engine.native_arrays()["vectors"]
         self.quantizer = ScalarQuantizer.fit(vectors)
         self.codes = self.quantizer.encode(vectors)
 
@@ -83,19 +86,21 @@ class QuantizedIndex:
         return len(self._index) * self._index.dim * 4
 
     def batch_search(self, queries, k: int = 10, ef: int | None = None, rerank: int = 0):
-        """``max(R, k)`` candidates from the codes traversal (``ef`` raised to at least that),
-        then, with ``rerank = R``, the best ``k`` by exact distance, ties to the lower id."""
-        if len(self._index) != self.codes.shape[0]:
+        """``k`` nearest neighbours per query. The traversal collects ``fetch = max(rerank, k)``
+        candidates (capped at the index size) with ef raised to at least ``fetch``; with
+        ``rerank > 0`` the best ``k`` of them by exact float64 distance are returned (ties to
+        the lower id), else the first ``k`` with their code distances mapped back."""
+        if self.codes.shape[0] != len(self._index):
             raise ValueError("the index changed after quantization; build a new QuantizedIndex")
-        k_valid, ef_valid = self._index._check_search_args(k, ef)
-        r = check_int(rerank, "rerank", minimum=0)
+        req = self._index._request(k, ef, None)
+        r = checks.integer(rerank, "rerank", minimum=0)
+        rows = self._index._prepare(queries)
         metric = self._index.metric
-        prepared = self._index._prepare(queries)
-        fetch = min(max(r, k_valid), len(self._index))
-        ids, code_dist = self._engine.batch_search_codes(
-            self.codes, self.quantizer.transform_queries(prepared, metric), fetch,
-            max(ef_valid, fetch),
-        )
-        if r == 0:
-            return ids, self.quantizer.to_distance(code_dist, prepared, metric)
-        return self._engine.rerank(prepared, ids, k_valid)
+        fetch = min(len(self._index), max(r, req.k))
+        code_q = self.quantizer.transform_queries(rows, metric)
+        engine = self._index._engine
+        if r:
+            return engine.quantized_search(self.codes, code_q, rows, req.k, fetch,
+                                           max(req.ef, fetch))
+        ids, d = engine.batch_search_codes(self.codes, code_q, fetch, max(req.ef, fetch))
+        return ids, self.quantizer.to_distance(d, rows, metric)

@@ -12,6 +12,7 @@
  *
  * Reference interfaces replaced (paths relative to the hnsw-engine repository):
  *   hm_open            NumbaEngine.attach_native      src/hnsw_engine/numba_engine.py:827-868
+ *   hm_check_graph     _validate_native               src/hnsw_engine/numba_engine.py:878-927
  *   hm_search          NumbaEngine.batch_search       src/hnsw_engine/numba_engine.py:652-672
  *                      (-> _search_batch :422-503, _search_layer :98-148,
  *                          _search_layer0_filtered :151-198)
@@ -91,6 +92,13 @@ typedef struct {
  * as visited, so results do not change). When every vector value is an integer in 0..255 an
  * exact uint8 copy is kept as well. */
 int hm_open(const hm_native_graph *graph, int device, hm_index **out);
+
+/* The reference's structural checks of native arrays before use (the kernels do not
+ * bounds-check): levels, entry point, link counts and ids on layer 0 and the upper layers, the
+ * tiling of upper rows by the nodes' offsets, and upper links to nodes present on that layer.
+ * HM_EINVAL with the reference's message otherwise. Host only.
+ *   replaces _validate_native / _check_links   src/hnsw_engine/numba_engine.py:878-927 */
+int hm_check_graph(const hm_native_graph *graph);
 void hm_close(hm_index *index);
 
 /* 1 if the index keeps the exact 8-bit copy (integer-valued 0..255 vectors, metric l2/ip). */
@@ -153,6 +161,18 @@ int hm_attach_codes(hm_index *index, const uint8_t *codes);
 /* Algorithm 5 over the codes with float64 code-space queries (nq, dim), unfiltered. */
 int hm_search_codes(hm_index *index, const double *queries, const hm_search_opts *opts,
                     int64_t *ids, double *dists, int64_t *dcount, int64_t *hops);
+
+/* QuantizedIndex.batch_search with rerank > 0 in one call: the codes traversal of
+ * hm_search_codes (opts: uniform ef, k_all = kstride = fetch, no filter) keeps its fetch
+ * candidates in HBM, the exact re-rank of hm_rerank scores them there, and only the best k per
+ * query come back: ids int64 / dists float64 (nq, k); dcount / hops per query as in
+ * hm_search, or null. code_queries: float64 (nq, dim) code-space queries; queries: the same
+ * queries prepared as float32 (nq, dim), for the exact distances.
+ *   replaces QuantizedIndex.batch_search   src/hnsw_engine/quantization.py:114-138
+ *   (batch_search_codes + rerank, numba_engine.py:674-713, without the host round trip) */
+int hm_search_codes_rerank(hm_index *index, const double *code_queries, const float *queries,
+                           const hm_search_opts *opts, int32_t k, int64_t *ids, double *dists,
+                           int64_t *dcount, int64_t *hops);
 
 /* Exact float64 distances of each row's candidate ids (-1 skipped) to the float32 vectors,
  * best k by (distance, id); -1 / +inf padding. candidates: int64 (nq, fetch). */

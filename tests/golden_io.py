@@ -6,7 +6,7 @@ from pathlib import Path
 
 import numpy as np
 
-from hnsw_amd.persistence import native_from_csr
+from hnsw_amd.graph_io import native_from_csr
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 

@@ -12,7 +12,7 @@ import pytest
 import golden_io
 from hnsw_amd import _native
 from hnsw_amd.engine import MI355XEngine
-from hnsw_amd.persistence import GraphState, diff_states
+from hnsw_amd.graph_io import GraphState, diff_states
 
 HEADER = Path(__file__).resolve().parents[1] / "include" / "hnsw_mi355x.h"
 

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from hnsw_amd import HNSWIndex
-from hnsw_amd.persistence import GraphState, diff_states, load_state, save_state, validate_state
+from hnsw_amd.graph_io import GraphState, diff_states, load_state, save_state, validate_state
 from hnsw_amd.synthetic import clustered_u8
 
 
@@ -57,7 +57,7 @@ def test_reference_written_snapshot_loads(tmp_path) -> None:
     "numba") loads into the MI355X engine."""
     index = _index()
     state = index.graph_state()
-    params = dict(index._params(), engine="numba")
+    params = dict(index._meta()["params"], engine="numba")
     arrays = {"meta": np.array(json.dumps({"format_version": 1, "params": params,
                                            "rng_state": index._rng.bit_generator.state})),
               "vectors": state.vectors, "levels": state.levels,

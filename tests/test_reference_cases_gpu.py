@@ -217,3 +217,40 @@ def test_complete_graph_search_equals_brute_force(data, metric, mask_seed, filte
     np.testing.assert_array_equal(ids[:, :kk], subset[t_ids])
     np.testing.assert_array_equal(dist[:, :kk], t_d)
     assert (ids[:, kk:] == -1).all()
+
+
+def test_native_round_trip_searches_identically(tmp_path) -> None:
+    """A saved index reopened from read-only memory maps searches exactly as the original
+    (``tests/test_numba_engine.py:237-253``), with and without a filter."""
+    data, queries = clustered_u8(2500, 8, 40, n_clusters=10, seed=1)
+    index = HNSWIndex(8, M=6, ef_construction=40, seed=3)
+    index.add(data)
+    index.save_native(tmp_path / "idx")
+    opened = HNSWIndex.load_native(tmp_path / "idx")
+    assert not opened._engine.native_arrays()["links0"].flags.writeable
+    allowed = np.random.default_rng(0).random(2500) < 0.2
+    for mask in (None, allowed):
+        print(f"case native round trip masked={mask is not None}", flush=True)
+        for a, b in zip(index.batch_search(queries, 10, ef=40, filter=mask),
+                        opened.batch_search(queries, 10, ef=40, filter=mask)):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_ef_below_k_is_raised_to_k() -> None:
+    """``ef = max(ef, k)`` (``index.py:294-299``): a search with ef < k is the search with
+    ef = k, counters included."""
+    data, queries = clustered_u8(3000, 16, 50, n_clusters=10, seed=5)
+    index = HNSWIndex(16, M=8, ef_construction=60, seed=2)
+    index.add(data)
+    print("case ef below k", flush=True)
+    index.reset_counters()
+    low = index.batch_search(queries, k=30, ef=5)
+    c_low = (index.distance_computations, index.hops)
+    index.reset_counters()
+    same = index.batch_search(queries, k=30, ef=30)
+    assert c_low == (index.distance_computations, index.hops)
+    for a, b in zip(low, same):
+        np.testing.assert_array_equal(a, b)
+    want = oracle.search_batch(index._engine.native_arrays(), index.entry_point, index.max_level,
+                               "l2", queries, 30, 30)
+    np.testing.assert_array_equal(low[0], want[0])
