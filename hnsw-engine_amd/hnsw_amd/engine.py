@@ -18,7 +18,14 @@ from typing import Any
 import numpy as np
 
 from . import _native
-from .graph_io import STORAGE, GraphState, native_from_csr, state_from_native
+from .graph_io import (
+    STORAGE,
+    GraphState,
+    check_native_graph,
+    check_native_layout,
+    native_from_csr,
+    state_from_native,
+)
 
 _INITIAL_CAPACITY = 1024
 
@@ -298,8 +305,15 @@ class MI355XEngine:
             "count_up": self._count_up[:rows],
         }
 
-    def attach_native(self, arrays: dict[str, np.ndarray], entry_point: int, max_level: int) -> None:
-        """Use validated native arrays (possibly read-only maps) in place on the host side."""
+    def attach_native(self, arrays: dict[str, np.ndarray], entry_point: int, max_level: int,
+                      *, validate: bool = True) -> None:
+        """Use native arrays (possibly read-only maps) in place on the host side, as the
+        reference's ``attach_native`` does (``numba_engine.py:827-868``): dtypes and shapes are
+        always checked, the links too unless ``validate`` is False (files you wrote yourself);
+        the first insertion copies them, so the arrays are never written."""
+        check_native_layout(arrays, self.dim, self.M)
+        if validate:
+            check_native_graph(arrays, entry_point, max_level, self.M, self.metric)
         for name in STORAGE:
             setattr(self, f"_{name}", arrays[name])
         self._size = arrays["vectors"].shape[0]

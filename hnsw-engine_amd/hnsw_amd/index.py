@@ -17,22 +17,13 @@ from __future__ import annotations
 
 import math
 from dataclasses import asdict, dataclass, replace
-from pathlib import Path
 from typing import Any, NamedTuple
 
 import numpy as np
 
 from . import checks
 from .engine import MI355XEngine
-from .graph_io import (
-    GraphState,
-    check_native_graph,
-    check_native_layout,
-    load_native,
-    load_state,
-    save_native,
-    save_state,
-)
+from .graph_io import GraphState, load_native, load_state, save_native, save_state
 from .metrics import METRICS, prepare
 
 ENGINES = ("mi355x",)
@@ -298,9 +289,6 @@ class HNSWIndex:
         reads it (skip it only for files you wrote yourself)."""
         meta, arrays = load_native(directory, mmap=mmap)
         index = cls._reopen(meta, device)
-        entry, top = int(meta["entry_point"]), int(meta["max_level"])
-        check_native_layout(arrays, index.dim, index.M)
-        if validate:
-            check_native_graph(arrays, entry, top, index.M, index.metric)
-        index._engine.attach_native(arrays, entry, top)
+        index._engine.attach_native(arrays, int(meta["entry_point"]), int(meta["max_level"]),
+                                    validate=validate)
         return index
