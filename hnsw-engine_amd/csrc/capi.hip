@@ -54,6 +54,7 @@ struct Batch {
     double *dists = nullptr;
     int32_t *dc = nullptr, *hops = nullptr, *fail = nullptr;
     bool learned = false;  // use the handle's stop model (register-beam path only)
+    const int32_t *subset = nullptr;  // device: queue position -> query index (a kernel class)
     // traces (register-beam path only)
     float *trace = nullptr;
     int trace_steps = 0;
@@ -92,13 +93,17 @@ hipError_t static_scratch_of(int device, void **out) {
 // clears the whole bitset).
 constexpr int kVisitedList = 4096;
 
-// Capacities of the on-chip state for a batch whose largest max(ef, k) is max_ef; nr: R
-// entries per lane of the register beam.
-void plan(SearchArgs &a, int mode, const hm_index *h, int max_ef, bool fast, int nr) {
+// Layer-0 kernels a launch can take (decided per launch; none changes results).
+enum Kind { kGeneral = 0, kFast = 1, kBeam = 2 };
+
+// Capacities of the on-chip state for a launch whose largest max(ef, k) is max_ef; nr: R
+// entries per lane of the register beams.
+void plan(SearchArgs &a, int mode, bool filt, const hm_index *h, int max_ef, int kind, int nr) {
     max_ef = std::max(1, max_ef);
-    a.fast = fast ? 1 : 0;
+    a.fast = kind == kFast ? 1 : 0;
+    a.beam = kind == kBeam ? 1 : 0;
     a.nr = nr;
-    if (fast) {
+    if (kind != kGeneral) {
         // 4096 slots (8 KB of 16-bit slots for up to 4M ids). Measured on 10,000 queries over
         // 1M x 128 with the longest-first schedule, against 2048 slots (which fit more waves per
         // CU but send 2-10 % of the layer searches to the global bitset): ef 16 / 32 / 48 / 64 /
@@ -108,7 +113,12 @@ void plan(SearchArgs &a, int mode, const hm_index *h, int max_ef, bool fast, int
         a.h_log2 = h->tune_fast_hlog ? h->tune_fast_hlog : (max_ef <= 8 ? 10 : 12);
         a.r_cap = 64 * nr;
         a.c_cap = 64;
-        a.lds_per_wave = lds_bytes_fast(mode, h->dim, h->dimp, a.h_log2, a.id_bits, nr);
+        if (kind == kFast) {
+            a.lds_per_wave = lds_bytes_fast(mode, h->dim, h->dimp, a.h_log2, a.id_bits, nr);
+        } else {
+            a.nc = filt ? 4 : 1;
+            a.lds_per_wave = lds_bytes_beam(mode, h->dim, h->dimp, a.h_log2, a.id_bits, nr, a.nc);
+        }
         return;
     }
     a.r_cap = (int)round_up(max_ef, 8);
@@ -117,6 +127,11 @@ void plan(SearchArgs &a, int mode, const hm_index *h, int max_ef, bool fast, int
                             : std::min(12, std::max(10, ceil_log2((int64_t)max_ef * 64)));
     a.lds_per_wave =
         lds_bytes_per_wave(mode, h->dim, h->dimp, a.r_cap, a.c_cap, a.h_log2, a.id_bits);
+}
+
+// R entries per lane a register beam needs for max(ef, k) up to max_ef (0: more than 512).
+int beam_width(int max_ef) {
+    return max_ef <= 64 ? 1 : max_ef <= 128 ? 2 : max_ef <= 256 ? 4 : max_ef <= 512 ? 8 : 0;
 }
 
 // Where a launch's scratch comes from: the handle's grow-only buffers (allocated on demand), or
@@ -179,19 +194,22 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap,
     a.mask_words = b.mask_words;
     a.mask_idx = b.midx;
     a.mask_all = b.mask_all;
-    const bool fast = allow_fast && !filt && b.max_ef <= 64 && b.kstride <= 64 && mode != kSeqU8;
-    // ef 65..512 on the constant shape: the register beam with two, four or eight entries per
-    // lane.
-    const int wide_nr = b.max_ef <= 128 && b.kstride <= 128   ? 2
-                        : b.max_ef <= 256 && b.kstride <= 256 ? 4
-                        : b.max_ef <= 512 && b.kstride <= 512 ? 8 : 0;
+    // Which layer-0 kernel: unfiltered max(ef, k) <= 64 the register beam (the packed form on
+    // the constant shape); unfiltered 65..512 on the constant shape the packed register beam with
+    // 2, 4 or 8 entries per lane; every other batch up to 512 the general register beam
+    // (filters, codes, other shapes); beyond 512, or a rerun after a spill overflow, the general
+    // kernel.
+    const int nr_ef = beam_width(b.max_ef);
+    const bool fast = allow_fast && !filt && b.max_ef <= 64 && mode != kSeqU8;
     const int wide_hlog = h->tune_fast_hlog ? h->tune_fast_hlog : 12;
-    const bool wide = allow_fast && !filt && !fast && !b.trace && mode == kExactU8 &&
-                      wide_nr > 0 && h->w0 == 32 && h->w1 == 16 && h->dimp == 128 &&
+    const bool wide = allow_fast && !filt && !fast && !b.trace && mode == kExactU8 && nr_ef > 0 &&
+                      h->w0 == 32 && h->w1 == 16 && h->dimp == 128 &&
                       a.id_bits - wide_hlog <= 10;
-    plan(a, mode, h, b.max_ef,
-         fast || wide || (allow_fast && !filt && b.max_ef <= 64 && b.kstride <= 64),
-         wide ? wide_nr : 1);
+    const bool fast_any = allow_fast && !filt && b.max_ef <= 64;
+    const int kind = fast || wide || fast_any ? kFast
+                     : allow_fast && !b.trace && nr_ef > 0 ? kBeam
+                                                           : kGeneral;
+    plan(a, mode, filt, h, b.max_ef, kind, kind == kFast ? (wide ? nr_ef : 1) : nr_ef);
     if (fast && b.trace) {
         a.trace = b.trace;
         a.trace_steps = b.trace_steps;
@@ -213,8 +231,9 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap,
     if (max_slots > 0) slots = std::min(slots, max_slots);
     a.bitset_words = (h->n + 31) / 32;
     a.vlist_cap = kVisitedList;
-    // The register beams never spill candidates.
+    // The register beams of kind kFast never spill candidates.
     if (a.fast) spill_cap = 0;
+    a.subset = b.subset;
     a.spill_cap = spill_cap;
     if (fixed) {
         Carver &c = *fixed;
@@ -227,7 +246,7 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap,
         uint64_t *pre_key = nullptr;
         uint32_t *pre_id = nullptr;
         int32_t *pre_dc = nullptr, *pre_hops = nullptr, *order = nullptr;
-        if (a.fast && two_phase_shape(h, mode) && !a.trace && h->tune_schedule == 0) {
+        if ((a.fast || a.beam) && two_phase_shape(h, mode) && !a.trace && h->tune_schedule == 0) {
             pre_key = c.take<uint64_t>(b.nq);
             pre_id = c.take<uint32_t>(b.nq);
             pre_dc = c.take<int32_t>(b.nq);
@@ -287,7 +306,8 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap,
     // layer 0 with the queries whose layer-0 entry lies farthest from them first, which
     // shortens the launch's tail. Measured on 10,000 queries over 1M x 128 against one launch:
     // ef 32 / 48 / 64 / 96 -12 / -17 / -16 / -14 %, ids identical.
-    if (a.fast && two_phase_shape(h, mode) && !a.trace && b.nq > slots && h->tune_schedule == 0) {
+    if ((a.fast || a.beam) && two_phase_shape(h, mode) && !a.trace && b.nq > slots &&
+        h->tune_schedule == 0) {
         a.descent_cap = h->tune_descent_cap > 0 ? h->tune_descent_cap : kDescentMaxCap;
         if (!fixed) {
             HM_TRY(h->pre_key.ensure(b.nq), "schedule allocation");
@@ -433,8 +453,48 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
     b.dc = h->dc.p;
     b.hops = h->hops.p;
     b.fail = h->fail.p;
-    st = launch(h, mode, filt, b, std::min<int64_t>(h->n, kDefaultSpill), 0, s);
-    if (st) return st;
+    const int64_t spill = std::min<int64_t>(h->n, kDefaultSpill);
+    // Queries that need different kernels (per-query ef; filtered and unfiltered ones) run as
+    // one launch per kernel class, each over its own subset of the batch: class = (filtered,
+    // register-beam width 1 / 2 / 4 / 8, or the general kernel beyond ef 512).
+    std::vector<int32_t> cls(nq);
+    int present = 0;
+    for (int64_t i = 0; i < nq; ++i) {
+        const bool fq = filt && (o->mask_index ? o->mask_index[i] >= 0 : o->mask_all >= 0);
+        const int w = beam_width(efv.empty() ? b.ef_all : efv[i]);
+        cls[i] = (fq ? 5 : 0) + (w == 0 ? 4 : w == 1 ? 0 : w == 2 ? 1 : w == 4 ? 2 : 3);
+        present |= 1 << cls[i];
+    }
+    if (__builtin_popcount(present) <= 1) {
+        st = launch(h, mode, filt, b, spill, 0, s);
+        if (st) return st;
+    } else {
+        std::vector<int32_t> order;
+        order.reserve(nq);
+        std::vector<int64_t> start(11, 0);
+        for (int c = 0; c < 10; ++c) {
+            start[c] = (int64_t)order.size();
+            for (int64_t i = 0; i < nq; ++i)
+                if (cls[i] == c) order.push_back((int32_t)i);
+        }
+        start[10] = nq;
+        HM_TRY(h->subset.ensure(nq), "alloc");
+        HM_TRY(hipMemcpyAsync(h->subset.p, order.data(), nq * 4, hipMemcpyHostToDevice, s),
+               "copy subsets");
+        for (int c = 0; c < 10; ++c) {
+            const int64_t cnt = start[c + 1] - start[c];
+            if (cnt == 0) continue;
+            Batch bc = b;
+            bc.nq = cnt;
+            bc.subset = h->subset.p + start[c];
+            int mx = 0;
+            for (int64_t t = start[c]; t < start[c + 1]; ++t)
+                mx = std::max(mx, efv.empty() ? b.ef_all : efv[order[t]]);
+            bc.max_ef = mx;
+            st = launch(h, mode, filt && c >= 5, bc, spill, 0, s);
+            if (st) return st;
+        }
+    }
     std::vector<int32_t> failv(nq), dcv(nq), hv(nq);
     HM_TRY(hipMemcpyAsync(failv.data(), h->fail.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
     HM_TRY(hipStreamSynchronize(s), "search");

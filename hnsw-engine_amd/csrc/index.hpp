@@ -38,7 +38,7 @@ struct hm_index {
     // staging of the host-buffer entry points
     hm::DevBuf<float> q32;
     hm::DevBuf<double> q64;
-    hm::DevBuf<int32_t> kq, efq, midx, dc, hops, fail;
+    hm::DevBuf<int32_t> kq, efq, midx, dc, hops, fail, subset;
     hm::DevBuf<uint32_t> maskbits;
     hm::DevBuf<int64_t> ids, cand, rids;
     hm::DevBuf<double> dists, rdists;
@@ -60,7 +60,7 @@ struct hm_index {
     void release_all() {
         hm::DevBuf<int32_t> *i32[] = {&links0, &links_up, &upper, &queue,    &kq,   &efq,
                                       &midx,   &dc,       &hops,  &fail,     &pre_dc,
-                                      &pre_hops, &order};
+                                      &pre_hops, &order, &subset};
         for (auto *b : i32) b->release();
         vec.release();
         u8.release();

@@ -43,6 +43,9 @@ struct Lds {
     uint32_t *model;  // learned stop: the packed model, kModelWords words per lane
     uint32_t *hist;  // 256 bins for the refill's radix select
     uint64_t *misc;  // [0..1]: first spilled entry
+    // general register beam: merge / sort buffer of 64 max(nr, nc) entries
+    uint64_t *bufk;
+    uint32_t *bufi, *buff;
 };
 
 HM_DEV Lds carve(uint8_t *base, const SearchArgs &a, int mode) {
@@ -103,6 +106,25 @@ HM_DEV Lds carve_fast(uint8_t *base, const SearchArgs &a, int mode, bool model) 
         L.model = (uint32_t *)p;
         p += kModelLdsBytes;
     }
+    L.hash = (uint32_t *)p;
+    return L;
+}
+
+// Layout of the general register beam (layer0_beam).
+HM_DEV Lds carve_beam(uint8_t *base, const SearchArgs &a, int mode) {
+    auto up = [](int v) { return (v + 15) & ~15; };
+    Lds L = {};
+    uint8_t *p = base;
+    L.qb = p;
+    p += mode == kExactU8 ? up(a.dimp) : up(a.dim * 8);
+    L.nbi = (uint32_t *)p; p += 64 * 4;
+    L.nbacc = (int32_t *)p; p += 64 * 4;
+    const int b = 64 * (a.nr > a.nc ? a.nr : a.nc);
+    L.bufk = (uint64_t *)p; p += b * 8;
+    L.bufi = (uint32_t *)p; p += b * 4;
+    L.buff = (uint32_t *)p; p += b * 4;
+    L.hist = (uint32_t *)p; p += 256 * 4;
+    L.misc = (uint64_t *)p; p += 16;
     L.hash = (uint32_t *)p;
     return L;
 }
@@ -219,6 +241,50 @@ HM_DEV void exact_rows(const SearchArgs &a, const Lds &L, int nnew, int lg_rt) {
     }
 }
 
+// Partial sums of dist_exact for rows of any chunk count (784 dimensions: 49 chunks): 16 lanes
+// per row, 4 rows per pass, lane j summing chunks j, j + 16, j + 32, ...; the loads of two passes
+// (8 rows) are all in flight before the first is used. The group's first lane stores the row's
+// sum into L.nbacc.
+HM_DEV void exact_rows_strided(const SearchArgs &a, const Lds &L, int nnew) {
+    const int lane = lane_id();
+    const int j = lane & 15;
+    const int cpv = a.dimp >> 4;
+    const uint4 *qv = (const uint4 *)L.qb;
+    constexpr int kIn = 4, kPass = 2;
+    for (int v0 = 0; v0 < nnew; v0 += 4 * kPass) {
+        int32_t part[kPass] = {};
+        for (int c0 = 0; c0 < cpv; c0 += 16 * kIn) {
+            uint4 xs[kPass][kIn];
+#pragma unroll
+            for (int ps = 0; ps < kPass; ++ps) {
+                if (v0 + 4 * ps >= nnew) break;  // wave-uniform
+                const int v = min(v0 + 4 * ps + (lane >> 4), nnew - 1);
+                const uint4 *row = (const uint4 *)(a.vec_u8 + (size_t)L.nbi[v] * a.dimp);
+#pragma unroll
+                for (int t = 0; t < kIn; ++t) {
+                    const int c = c0 + j + 16 * t;
+                    if (c < cpv) xs[ps][t] = row[c];
+                }
+            }
+#pragma unroll
+            for (int ps = 0; ps < kPass; ++ps) {
+                if (v0 + 4 * ps >= nnew) break;
+#pragma unroll
+                for (int t = 0; t < kIn; ++t) {
+                    const int c = c0 + j + 16 * t;
+                    if (c < cpv) part[ps] += exact_partial(xs[ps][t], qv[c], a.metric);
+                }
+            }
+        }
+#pragma unroll
+        for (int ps = 0; ps < kPass; ++ps) {
+            const int v = v0 + 4 * ps + (lane >> 4);
+            const int32_t sum = group_sum(part[ps], 16);
+            if (j == 0 && v < nnew) L.nbacc[v] = sum;
+        }
+    }
+}
+
 // Exact integer distances of the nnew ids in L.nbi to the 8-bit query. Lanes 0..nnew-1 get
 // their key. Every partial sum is an exact integer, so any summation order equals the
 // reference's float64 sum.
@@ -261,15 +327,7 @@ HM_DEV uint64_t dist_exact(const SearchArgs &a, const Lds &L, int nnew, int32_t 
     } else if ((cpv & (cpv - 1)) == 0 && cpv <= 64) {
         exact_rows<-1>(a, L, nnew, __builtin_ctz(cpv));
     } else {
-        if (lane < nnew) L.nbacc[lane] = 0;
-        wave_sync();
-        const int total = nnew * cpv;
-        for (int t = lane; t < total; t += 64) {
-            int v = t / cpv, c = t - v * cpv;
-            uint32_t id = L.nbi[v];
-            uint4 x = *(const uint4 *)(a.vec_u8 + (size_t)id * a.dimp + (size_t)c * 16);
-            atomicAdd(&L.nbacc[v], exact_partial(x, qv[c], a.metric));
-        }
+        exact_rows_strided(a, L, nnew);
     }
     wave_sync();
     uint64_t key = kInfKey;
@@ -1521,10 +1579,438 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
     return true;
 }
 
+// ---------------------------------------------------------------------------- layer 0, general register beam
+
+// Entries of the general register beam. Exact 8-bit keys are below 2^32, so (key, id) packs into
+// one word whose order is the pair's order; a float64 key keeps its 64-bit image and travels
+// with the id.
+template <bool PACK> struct Ent;
+
+template <> struct Ent<true> {
+    uint64_t w;
+    static HM_DEV Ent inf() { return Ent{~0ull}; }
+    static HM_DEV Ent make(uint64_t k, uint32_t i) { return Ent{(k << 32) | i}; }
+    HM_DEV uint64_t key() const { return w >> 32; }
+    HM_DEV uint32_t id() const { return (uint32_t)w; }
+    HM_DEV bool lt(const Ent &o) const { return w < o.w; }
+    HM_DEV Ent at(int l) const { return Ent{readlane_u64(w, l)}; }  // lane l's entry, uniform
+    HM_DEV void put(const Lds &L, int p) const { L.bufk[p] = w; }
+    static HM_DEV Ent get(const Lds &L, int p) { return Ent{L.bufk[p]}; }
+};
+
+template <> struct Ent<false> {
+    uint64_t k;
+    uint32_t i;
+    static HM_DEV Ent inf() { return Ent{kInfKey, kInfId}; }
+    static HM_DEV Ent make(uint64_t k, uint32_t i) { return Ent{k, i}; }
+    HM_DEV uint64_t key() const { return k; }
+    HM_DEV uint32_t id() const { return i; }
+    HM_DEV bool lt(const Ent &o) const { return k < o.k || (k == o.k && i < o.i); }
+    HM_DEV Ent at(int l) const { return Ent{readlane_u64(k, l), readlane_u32(i, l)}; }
+    HM_DEV void put(const Lds &L, int p) const {
+        L.bufk[p] = k;
+        L.bufi[p] = i;
+    }
+    static HM_DEV Ent get(const Lds &L, int p) { return Ent{L.bufk[p], L.bufi[p]}; }
+};
+
+template <int N, class E> HM_DEV E ent_slot(const E (&v)[N], int s) {
+    E r = v[0];
+#pragma unroll
+    for (int i = 1; i < N; ++i)
+        if (s == i) r = v[i];
+    return r;
+}
+
+// Lanes of slot s that hold one of the first n entries of a 64-per-slot register array.
+HM_DEV uint64_t slot_mask(int n, int s) {
+    const int c = min(max(n - 64 * s, 0), 64);
+    return c >= 64 ? ~0ull : ((1ull << c) - 1ull);
+}
+
+// Layer 0 with R and the other candidates in registers: any metric and distance kernel,
+// filtered or not, ef <= 64 NR (numba_engine.py:98-148 and, with FILT, :151-198).
+//
+// R, the results (allowed nodes only), is sorted (key, id) with entry e in lane e % 64 of slot
+// e / 64; a per-slot mask marks the entries already expanded. Every other node that the
+// reference would still hold as a candidate is in C: the accepted nodes that are not allowed
+// and the unexpanded entries evicted from R. C is sorted too, NC slots in registers, of which
+// the first ch are expanded (candidates leave C in order); what does not fit goes to a per-wave
+// spill in global memory, every spilled entry larger than every entry of C, and comes back by a
+// radix select when C runs dry. Each step expands the smaller of R's first unexpanded entry and
+// C's first unexpanded one, and stops when none is left or, with R full, when that candidate
+// lies beyond R's worst key: the reference's pop order and break rule. Once R is full a C entry
+// beyond the worst key can never be expanded, so merges drop such entries.
+//
+// One expansion, in link order (rows wider than 64 links 64 at a time): acceptance of new
+// neighbour j iff #{r in R: key_r <= key_j} + #{earlier new i (allowed, when filtered): key_i <=
+// key_j} < ef (once R is full only keys below its worst take part); R becomes the ef smallest of
+// R and the accepted allowed entries; C gains the accepted entries that are not allowed and the
+// unexpanded entries pushed out of R.
+template <int MODE, bool FILT, int NR, int NC, int SH>
+HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q, int slot,
+                        uint64_t entry_k, uint32_t entry_id, uint64_t *rk_out, uint32_t *ri_out,
+                        int &rn) {
+    using E = Ent<MODE == kExactU8>;
+    constexpr int kC = 64 * NC;
+    const int lane = lane_id();
+    const int ef = Q.ef;
+    uint64_t *const sk = a.spill_key + (size_t)slot * a.spill_cap;
+    uint32_t *const si = a.spill_id + (size_t)slot * a.spill_cap;
+    visited_reset(a, L, V, a.h_log2);
+    visit_lanes(L, V, entry_id, lane == 0);
+    V.count = 1;
+    E r[NR], c[NC];
+    uint64_t rexp[NR];
+#pragma unroll
+    for (int s = 0; s < NR; ++s) {
+        r[s] = E::inf();
+        rexp[s] = 0;
+    }
+#pragma unroll
+    for (int s = 0; s < NC; ++s) c[s] = E::inf();
+    int cn = 0, ch = 0;  // C holds entries [0, cn) of which [0, ch) were expanded
+    rn = 0;
+    {
+        const E e0 = E::make(entry_k, entry_id);
+        if (!FILT || allowed(Q, entry_id)) {
+            if (lane == 0) r[0] = e0;
+            rn = 1;
+        } else {
+            if (lane == 0) c[0] = e0;
+            cn = 1;
+        }
+    }
+    int spn = 0;          // spilled candidates
+    E smin = E::inf();    // the smallest spilled one
+    auto worst_key = [&]() -> uint64_t {  // R's worst key when R is full, else +inf
+        return rn == ef ? ent_slot<NR>(r, (ef - 1) >> 6).at((ef - 1) & 63).key() : kInfKey;
+    };
+    for (;;) {
+        if (ch == cn && spn > 0) {
+            // C ran dry: prune the spill, then bring back its smallest kC entries.
+            __threadfence();
+            const uint64_t wk = worst_key();
+            int w = 0;
+            for (int b = 0; b < spn; b += 64) {
+                const int j = b + lane;
+                const bool v = j < spn;
+                const uint64_t k = v ? sk[j] : kInfKey;
+                const uint32_t id = v ? si[j] : kInfId;
+                const bool kp = v && k <= wk;
+                const uint64_t m = ballot(kp);
+                if (kp) {
+                    sk[w + rank_below(m)] = k;
+                    si[w + rank_below(m)] = id;
+                }
+                w += popc(m);
+            }
+            __threadfence();
+            spn = 0;
+            smin = E::inf();
+            cn = ch = 0;
+            if (w > 0) {
+                uint64_t bk = kInfKey;
+                uint32_t bi = kInfId;
+                if (w > kC) spill_boundary(L, sk, si, w, kC, bk, bi);
+                int nc = 0, nw = 0;
+                uint64_t mk = kInfKey;
+                uint32_t mi = kInfId;
+                for (int b = 0; b < w; b += 64) {
+                    const int j = b + lane;
+                    const bool v = j < w;
+                    const uint64_t k = v ? sk[j] : kInfKey;
+                    const uint32_t id = v ? si[j] : kInfId;
+                    const bool sel = v && less(k, id, bk, bi);
+                    const bool rest = v && !sel;
+                    const uint64_t m1 = ballot(sel), m2 = ballot(rest);
+                    if (sel) {
+                        L.bufk[nc + rank_below(m1)] = k;
+                        L.bufi[nc + rank_below(m1)] = id;
+                    }
+                    if (rest) {
+                        sk[nw + rank_below(m2)] = k;
+                        si[nw + rank_below(m2)] = id;
+                        if (less(k, id, mk, mi)) {
+                            mk = k;
+                            mi = id;
+                        }
+                    }
+                    nc += popc(m1);
+                    nw += popc(m2);
+                }
+                __threadfence();
+                if (nc == 0) return false;  // cannot happen: the select takes 1..kC entries
+                wave_min_pair(mk, mi);
+                smin = nw > 0 ? E::make(mk, mi) : E::inf();
+                spn = nw;
+                int p2 = 1;
+                while (p2 < nc) p2 <<= 1;
+                bitonic_sort(L.bufk, L.bufi, nc, p2);
+#pragma unroll
+                for (int s = 0; s < NC; ++s) {
+                    const int e = lane + 64 * s;
+                    c[s] = e < nc ? E::make(L.bufk[e], L.bufi[e]) : E::inf();
+                }
+                cn = nc;
+                wave_sync();
+            }
+        }
+        // The next candidate: R's first unexpanded entry or C's, whichever is smaller.
+        E er = E::inf();
+        int rs = -1, rp = 0;
+#pragma unroll
+        for (int s = 0; s < NR; ++s) {
+            const uint64_t live = slot_mask(rn, s) & ~rexp[s];
+            if (rs < 0 && live) {
+                rp = __ffsll((long long)live) - 1;
+                er = r[s].at(rp);
+                rs = s;
+            }
+        }
+        const E ec = ch < cn ? ent_slot<NC>(c, ch >> 6).at(ch & 63) : E::inf();
+        if (rs < 0 && ch >= cn) break;
+        E cur;
+        if (rs >= 0 && (ch >= cn || er.lt(ec))) {
+            cur = er;
+#pragma unroll
+            for (int s = 0; s < NR; ++s)
+                if (s == rs) rexp[s] |= 1ull << rp;
+        } else {
+            cur = ec;
+            ch += 1;
+            if (cur.key() > worst_key()) break;  // the reference's break (R entries never are)
+        }
+        const uint32_t cid = cur.id();
+        Q.hops += 1;
+        const int w0 = w0_of<SH>(a);
+        for (int base = 0; base < w0; base += 64) {
+            const int j = base + lane;
+            const int32_t nb = j < w0 ? a.links0[(size_t)cid * w0 + j] : -1;
+            visited_make_room(L, V, min(64, w0 - base));
+            const bool isnew = visit_lanes(L, V, (uint32_t)nb, nb >= 0);
+            visited_note(V, isnew, (uint32_t)nb);
+            const uint64_t nm = ballot(isnew);
+            const int nnew = popc(nm);
+            if (nnew == 0) continue;
+            if (isnew) L.nbi[rank_below(nm)] = (uint32_t)nb;
+            wave_sync();
+            V.count += nnew;
+            const uint64_t ki = distances<MODE, SH>(a, L, nnew, Q.qq);
+            const uint32_t ii = lane < nnew ? L.nbi[lane] : kInfId;
+            Q.dcount += nnew;
+            const E ne = lane < nnew ? E::make(ki, ii) : E::inf();
+            const uint64_t all_new = nnew >= 64 ? ~0ull : ((1ull << nnew) - 1ull);
+            const uint64_t alm = FILT ? ballot(lane < nnew && allowed(Q, ii)) : all_new;
+            // Acceptance (see above); a neighbour at or above the worst key of a full R is
+            // rejected and counts for no neighbour below that key.
+            const uint64_t wk = worst_key();
+            const uint64_t part = rn == ef ? ballot(lane < nnew && ki < wk) : all_new;
+            uint64_t rvalid[NR];
+#pragma unroll
+            for (int s = 0; s < NR; ++s) rvalid[s] = slot_mask(rn, s);
+            int pos_r[NR];
+#pragma unroll
+            for (int s = 0; s < NR; ++s) pos_r[s] = lane + 64 * s;
+            int r_below = 0, a_below = 0;
+            uint64_t am = 0, aam = 0;
+            for (uint64_t m = part; m; m &= m - 1ull) {
+                const int q = __ffsll((long long)m) - 1;
+                const E eq = ne.at(q);
+                const uint64_t kq = eq.key();
+                int cnt = popc(ballot(ki <= kq) & part & alm & ((1ull << q) - 1ull));
+#pragma unroll
+                for (int s = 0; s < NR; ++s) cnt += popc(ballot(r[s].key() <= kq) & rvalid[s]);
+                if (cnt >= ef) continue;
+                am |= 1ull << q;
+                if (FILT && !((alm >> q) & 1ull)) continue;
+                aam |= 1ull << q;
+                int below = 0;
+#pragma unroll
+                for (int s = 0; s < NR; ++s) {
+                    if (eq.lt(r[s])) pos_r[s] += 1;
+                    below += popc(ballot(r[s].lt(eq)) & rvalid[s]);
+                }
+                if (lane == q) r_below = below;
+                if (eq.lt(ne)) a_below += 1;
+            }
+            if (am == 0) continue;
+            const bool acc = (am >> lane) & 1ull;
+            const bool acc_al = (aam >> lane) & 1ull;
+            const int naa = popc(aam);
+            const int pos_a = r_below + a_below;
+            // R <- the ef smallest of R and the accepted allowed entries.
+            const int nrn = min(ef, rn + naa);
+            E nr[NR];
+            uint64_t nexp[NR];
+            bool ev_r[NR];
+            if (naa > 0) {
+#pragma unroll
+                for (int s = 0; s < NR; ++s) {
+                    if (((rvalid[s] >> lane) & 1ull) && pos_r[s] < ef) {
+                        r[s].put(L, pos_r[s]);
+                        L.buff[pos_r[s]] = (uint32_t)((rexp[s] >> lane) & 1ull);
+                    }
+                }
+                if (acc_al && pos_a < ef) {
+                    ne.put(L, pos_a);
+                    L.buff[pos_a] = 0u;
+                }
+                wave_sync();
+#pragma unroll
+                for (int s = 0; s < NR; ++s) {
+                    const int e = lane + 64 * s;
+                    const bool in = e < nrn;
+                    nr[s] = in ? E::get(L, e) : E::inf();
+                    nexp[s] = ballot(in && L.buff[e] != 0u);
+                    ev_r[s] = ((rvalid[s] >> lane) & 1ull) && pos_r[s] >= ef &&
+                              !((rexp[s] >> lane) & 1ull);
+                }
+                wave_sync();
+            } else {
+#pragma unroll
+                for (int s = 0; s < NR; ++s) {
+                    nr[s] = r[s];
+                    nexp[s] = rexp[s];
+                    ev_r[s] = false;
+                }
+            }
+            // C <- C and the new candidates: accepted entries that are not allowed, and the
+            // unexpanded entries pushed out of R (the new ones among them included).
+            const bool s_new = (acc && !acc_al) || (acc_al && pos_a >= ef);
+            uint64_t sm_r[NR];
+            int ns = popc(ballot(s_new));
+#pragma unroll
+            for (int s = 0; s < NR; ++s) {
+                sm_r[s] = ballot(ev_r[s]);
+                ns += popc(sm_r[s]);
+            }
+            if (ns > 0) {
+                const uint64_t nwk =
+                    nrn == ef ? ent_slot<NR>(nr, (ef - 1) >> 6).at((ef - 1) & 63).key() : kInfKey;
+                uint64_t clive[NC];
+#pragma unroll
+                for (int s = 0; s < NC; ++s) clive[s] = slot_mask(cn, s) & ~slot_mask(ch, s);
+                int pc[NC];
+#pragma unroll
+                for (int s = 0; s < NC; ++s) pc[s] = lane + 64 * s - ch;
+                int hb_new = 0, xb_new = 0;
+                int hb_r[NR], xb_r[NR];
+#pragma unroll
+                for (int s = 0; s < NR; ++s) hb_r[s] = xb_r[s] = 0;
+                // one step per new candidate x: it moves the C entries above it up by one, counts
+                // the live C entries below it, and is counted by the new candidates above it
+                auto step = [&](const E &x, int q, int holder) {
+                    int hb = 0;
+#pragma unroll
+                    for (int s = 0; s < NC; ++s) {
+                        if (x.lt(c[s])) pc[s] += 1;
+                        hb += popc(ballot(c[s].lt(x)) & clive[s]);
+                    }
+                    if (holder < 0) {
+                        if (lane == q) hb_new = hb;
+                    }
+#pragma unroll
+                    for (int s = 0; s < NR; ++s)
+                        if (holder == s && lane == q) hb_r[s] = hb;
+                    if (x.lt(ne)) xb_new += 1;
+#pragma unroll
+                    for (int s = 0; s < NR; ++s)
+                        if (x.lt(r[s])) xb_r[s] += 1;
+                };
+                for (uint64_t m = ballot(s_new); m; m &= m - 1ull) {
+                    const int q = __ffsll((long long)m) - 1;
+                    step(ne.at(q), q, -1);
+                }
+#pragma unroll
+                for (int s = 0; s < NR; ++s)
+                    for (uint64_t m = sm_r[s]; m; m &= m - 1ull) {
+                        const int q = __ffsll((long long)m) - 1;
+                        step(r[s].at(q), q, s);
+                    }
+                const int total = cn - ch + ns;
+                // sorted, so what lies beyond the new worst key (R full) is a suffix, and so is
+                // what is not below the smallest spilled entry
+                int cut_p = total, cut_s = total;
+                if (nrn == ef || spn > 0) {
+                    int np = 0, nsp = 0;
+#pragma unroll
+                    for (int s = 0; s < NC; ++s) {
+                        np += popc(ballot(c[s].key() <= nwk) & clive[s]);
+                        nsp += popc(ballot(c[s].lt(smin)) & clive[s]);
+                    }
+                    np += popc(ballot(s_new && ne.key() <= nwk));
+                    nsp += popc(ballot(s_new && ne.lt(smin)));
+#pragma unroll
+                    for (int s = 0; s < NR; ++s) {
+                        np += popc(ballot(r[s].key() <= nwk) & sm_r[s]);
+                        nsp += popc(ballot(r[s].lt(smin)) & sm_r[s]);
+                    }
+                    if (nrn == ef) cut_p = np;
+                    if (spn > 0) cut_s = nsp;
+                }
+                const int keep = min(kC, min(cut_p, cut_s));
+                const int nspill = cut_p - keep;
+                if (nspill > 0 && (int64_t)spn + nspill > a.spill_cap) return false;
+                // place every kept entry in the buffer and every spilled one in the spill; the
+                // entry at position keep is the smallest spilled this time
+                auto place = [&](const E &x, int p) {
+                    if (p < keep) {
+                        x.put(L, p);
+                    } else if (p < cut_p) {
+                        sk[spn + p - keep] = x.key();
+                        si[spn + p - keep] = x.id();
+                        if (p == keep) {
+                            L.misc[0] = x.key();
+                            L.misc[1] = x.id();
+                        }
+                    }
+                };
+#pragma unroll
+                for (int s = 0; s < NC; ++s)
+                    if ((clive[s] >> lane) & 1ull) place(c[s], pc[s]);
+                if (s_new) place(ne, hb_new + xb_new);
+#pragma unroll
+                for (int s = 0; s < NR; ++s)
+                    if ((sm_r[s] >> lane) & 1ull) place(r[s], hb_r[s] + xb_r[s]);
+                wave_sync();
+#pragma unroll
+                for (int s = 0; s < NC; ++s) {
+                    const int e = lane + 64 * s;
+                    c[s] = e < keep ? E::get(L, e) : E::inf();
+                }
+                if (nspill > 0) {
+                    const E first = E::make(L.misc[0], (uint32_t)L.misc[1]);
+                    if (first.lt(smin)) smin = first;
+                    spn += nspill;
+                }
+                cn = keep;
+                ch = 0;
+                wave_sync();
+            }
+#pragma unroll
+            for (int s = 0; s < NR; ++s) {
+                r[s] = nr[s];
+                rexp[s] = nexp[s];
+            }
+            rn = nrn;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NR; ++s) {
+        const bool in = lane + 64 * s < rn;
+        rk_out[s] = in ? r[s].key() : kInfKey;
+        ri_out[s] = in ? r[s].id() : kInfId;
+    }
+    return true;
+}
+
 // ---------------------------------------------------------------------------- one query
 
-template <int MODE, bool FILT, bool FAST, int AUX, int SH>
-HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int slot,
+// One query: qi indexes the batch's per-query arrays, p the schedule arrays (pre_*) of this
+// launch. BNR > 0: layer 0 on the general register beam with BNR entries per lane.
+template <int MODE, bool FILT, bool FAST, int AUX, int SH, int BNR = 0>
+HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int p, int slot,
                       const StopModel &M) {
     const int lane = lane_id();
     Query Q;
@@ -1557,11 +2043,11 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
     }
     uint64_t cur_k;
     uint32_t cur_id;
-    if (a.phase == 2 && a.pre_hops[qi] >= 0) {  // descend_kernel ran the descent
-        cur_k = a.pre_key[qi];
-        cur_id = a.pre_id[qi];
-        Q.dcount = a.pre_dc[qi];
-        Q.hops = a.pre_hops[qi];
+    if (a.phase == 2 && a.pre_hops[p] >= 0) {  // descend_kernel ran the descent
+        cur_k = a.pre_key[p];
+        cur_id = a.pre_id[p];
+        Q.dcount = a.pre_dc[p];
+        Q.hops = a.pre_hops[p];
         wave_sync();
     } else {
         if (lane == 0) L.nbi[0] = (uint32_t)a.entry;
@@ -1580,7 +2066,27 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
     double *od = a.out_d + (size_t)qi * a.kstride;
     int rn = 0;
     bool ok;
-    if constexpr (FAST) {
+    if constexpr (BNR > 0) {
+        uint64_t rk[BNR];
+        uint32_t ri[BNR];
+        ok = layer0_beam<MODE, FILT, BNR, FILT ? 4 : 1, SH>(a, L, V, Q, slot, cur_k, cur_id, rk,
+                                                           ri, rn);
+        if (!ok) rn = 0;
+        const int take = min(Q.k, rn);
+#pragma unroll
+        for (int s = 0; s < BNR; ++s) {  // lane i of slot s holds R[i + 64 s]
+            const int i = lane + 64 * s;
+            if (i < a.kstride) {
+                const bool own = i < take;
+                oi[i] = own ? (int64_t)ri[s] : -1;
+                od[i] = own ? decode<MODE>(rk[s], a.metric) : __builtin_huge_val();
+            }
+        }
+        for (int i = 64 * BNR + lane; i < a.kstride; i += 64) {  // k <= ef <= 64 BNR
+            oi[i] = -1;
+            od[i] = __builtin_huge_val();
+        }
+    } else if constexpr (FAST) {
         constexpr int NR = nr_of<SH>();
         uint64_t rk[NR];
         uint32_t ri[NR];
@@ -1591,13 +2097,17 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
         if (!ok) rn = 0;
         const int take = min(Q.k, rn);
 #pragma unroll
-        for (int s = 0; s < NR; ++s) {  // lane i of slot s holds R[i + 64 s]; kstride <= 64 NR
+        for (int s = 0; s < NR; ++s) {  // lane i of slot s holds R[i + 64 s]
             const int i = lane + 64 * s;
             if (i < a.kstride) {
                 const bool own = i < take;
                 oi[i] = own ? (int64_t)ri[s] : -1;
                 od[i] = own ? decode<MODE>(rk[s], a.metric) : __builtin_huge_val();
             }
+        }
+        for (int i = 64 * NR + lane; i < a.kstride; i += 64) {  // k <= ef <= 64 NR
+            oi[i] = -1;
+            od[i] = __builtin_huge_val();
         }
     } else {
         int rb = 0;
@@ -1622,11 +2132,13 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
     wave_sync();
 }
 
-template <int MODE, bool FILT, bool FAST, int AUX, int SH>
+template <int MODE, bool FILT, bool FAST, int AUX, int SH, int BNR = 0>
 HM_DEV void search_loop(const SearchArgs &a) {
     extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
     const int slot = blockIdx.x;
-    const Lds L = FAST ? carve_fast(lds_raw, a, MODE, AUX == 2) : carve(lds_raw, a, MODE);
+    const Lds L = BNR > 0 ? carve_beam(lds_raw, a, MODE)
+                  : FAST  ? carve_fast(lds_raw, a, MODE, AUX == 2)
+                          : carve(lds_raw, a, MODE);
     Visited V;
     V.bits = a.bitsets + (size_t)slot * a.bitset_words;
     V.list = a.vlist + (size_t)slot * a.vlist_cap;
@@ -1644,12 +2156,13 @@ HM_DEV void search_loop(const SearchArgs &a) {
     StopModel M = {};
     if constexpr (AUX == 2) load_model(a, L, M);
     for (;;) {
-        int qi = 0;
-        if (lane_id() == 0) qi = atomicAdd(a.queue, 1);
-        qi = uniform(shfl(qi, 0));
-        if (qi >= a.nq) break;
-        if (a.order) qi = uniform(a.order[qi]);
-        run_query<MODE, FILT, FAST, AUX, SH>(a, L, V, qi, slot, M);
+        int pos = 0;
+        if (lane_id() == 0) pos = atomicAdd(a.queue, 1);
+        pos = uniform(shfl(pos, 0));
+        if (pos >= a.nq) break;
+        const int p = a.order ? uniform(a.order[pos]) : pos;
+        const int qi = a.subset ? uniform(a.subset[p]) : p;
+        run_query<MODE, FILT, FAST, AUX, SH, BNR>(a, L, V, qi, p, slot, M);
     }
     // Leave the bitset clean for the next launch.
     if (V.dirty) visited_clear(a, V);
@@ -1676,6 +2189,14 @@ template <int MODE, int SH>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void search_stop_kernel(
     SearchArgs a) {
     search_loop<MODE, false, true, 2, SH>(a);
+}
+
+// General register beam (filters, codes, any shape): LDS allows 7-11 waves per CU, so no
+// occupancy target for the compiler beyond what the registers of the widest beam need.
+template <int MODE, bool FILT, int NR, int SH>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void search_beam_kernel(
+    SearchArgs a) {
+    search_loop<MODE, FILT, false, 0, SH, NR>(a);
 }
 
 __global__ void reset_queue_kernel(int32_t *q) { q[0] = 0; }
@@ -1747,9 +2268,10 @@ __global__ __launch_bounds__(256) void descend_kernel(SearchArgs a) {
     const int j = lane & 15;
     const int g = threadIdx.x >> 4;
     const int first = lane & 48;  // this query's first lane in the wave
-    const int qi = blockIdx.x * kDescentGroups + g;
-    const bool act = qi < a.nq;
-    const int qc = act ? qi : a.nq - 1;  // groups past the batch compute on a valid query
+    const int pi = blockIdx.x * kDescentGroups + g;  // position in the launch's schedule arrays
+    const bool act = pi < a.nq;
+    const int pc = act ? pi : a.nq - 1;  // groups past the batch compute on a valid query
+    const int qc = a.subset ? a.subset[pc] : pc;
     uint32_t *const tab = table[g];
 
     // Query bytes (lane j: bytes 8j..8j+7) and |q|^2, every lane of the block active.
@@ -1861,10 +2383,10 @@ __global__ __launch_bounds__(256) void descend_kernel(SearchArgs a) {
         }
     }
     if (act && j == 0) {
-        a.pre_key[qi] = flagged ? kInfKey : (uint64_t)cur_k;
-        a.pre_id[qi] = cur_id;
-        a.pre_dc[qi] = dcount;
-        a.pre_hops[qi] = flagged ? -1 : hops;
+        a.pre_key[pi] = flagged ? kInfKey : (uint64_t)cur_k;
+        a.pre_id[pi] = cur_id;
+        a.pre_dc[pi] = dcount;
+        a.pre_hops[pi] = flagged ? -1 : hops;
     }
 }
 
@@ -1882,6 +2404,18 @@ template __global__ void search_fast_kernel<kSeqF32, 0, kShapeAny>(SearchArgs);
 template __global__ void search_fast_kernel<kSeqU8, 0, kShapeAny>(SearchArgs);
 template __global__ void search_fast_kernel<kExactU8, 1, kShapeAny>(SearchArgs);
 template __global__ void search_fast_kernel<kSeqF32, 1, kShapeAny>(SearchArgs);
+#define HM_BEAM(MODE, FILT, SH)                                          \
+    template __global__ void search_beam_kernel<MODE, FILT, 1, SH>(SearchArgs); \
+    template __global__ void search_beam_kernel<MODE, FILT, 2, SH>(SearchArgs); \
+    template __global__ void search_beam_kernel<MODE, FILT, 4, SH>(SearchArgs); \
+    template __global__ void search_beam_kernel<MODE, FILT, 8, SH>(SearchArgs);
+HM_BEAM(kExactU8, true, kShapeAny)
+HM_BEAM(kExactU8, true, kShapeM16D128)
+HM_BEAM(kSeqF32, true, kShapeAny)
+HM_BEAM(kExactU8, false, kShapeAny)
+HM_BEAM(kSeqF32, false, kShapeAny)
+HM_BEAM(kSeqU8, false, kShapeAny)
+#undef HM_BEAM
 template __global__ void search_stop_kernel<kExactU8, kShapeAny>(SearchArgs);
 template __global__ void search_stop_kernel<kExactU8, kShapeM16D128>(SearchArgs);
 template __global__ void search_stop_kernel<kSeqF32, kShapeAny>(SearchArgs);
@@ -1907,6 +2441,23 @@ static KernelFn pick(int mode, bool filt, const SearchArgs &a) {
         if (mode != kExactU8) return search_stop_kernel<kSeqF32, kShapeAny>;
         return m16 ? search_stop_kernel<kExactU8, kShapeM16D128>
                    : search_stop_kernel<kExactU8, kShapeAny>;
+    }
+    if (a.beam) {
+#define HM_NR(MODE, FILT, SH)                                                     \
+    (a.nr == 1   ? search_beam_kernel<MODE, FILT, 1, SH>                          \
+     : a.nr == 2 ? search_beam_kernel<MODE, FILT, 2, SH>                          \
+     : a.nr == 4 ? search_beam_kernel<MODE, FILT, 4, SH>                          \
+                 : search_beam_kernel<MODE, FILT, 8, SH>)
+        if (filt) {
+            if (mode == kExactU8)
+                return shape_m16d128(a, mode) ? HM_NR(kExactU8, true, kShapeM16D128)
+                                              : HM_NR(kExactU8, true, kShapeAny);
+            return HM_NR(kSeqF32, true, kShapeAny);
+        }
+        if (mode == kExactU8) return HM_NR(kExactU8, false, kShapeAny);
+        if (mode == kSeqF32) return HM_NR(kSeqF32, false, kShapeAny);
+        return HM_NR(kSeqU8, false, kShapeAny);
+#undef HM_NR
     }
     if (fast && !filt) {
         if (a.nr == 2) return search_fast_kernel<kExactU8, 0, kShapeM16D128Wide>;

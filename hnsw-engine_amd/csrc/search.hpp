@@ -74,6 +74,11 @@ struct SearchArgs {
     int slots;
     int fast;  // 1: unfiltered, every ef <= 64 (<= 128 when nr = 2) -> the register beam
     int nr;    // register beam: R entries per lane (2: the constant shape with ef <= 128)
+    int beam;  // 1: the general register beam (filters, codes, any shape; ef <= 64 nr)
+    int nc;    // general register beam: held candidates per lane
+    // queue position (after `order`) -> query index; null: the identity. A batch split into
+    // kernel classes launches each class over its own subset.
+    const int32_t *subset;
     // learned stopping (register-beam kernel only): model parameters (layout below) and the
     // logit above which a beam stops; at least stop_tmin layer-0 expansions run first
     const float *stop_w;
@@ -128,6 +133,18 @@ inline int lds_bytes_fast(int mode, int dim, int dimp, int h_log2, int id_bits, 
     const int h0 = visited_hash_bytes(h_log2, id_bits), hu = visited_hash_bytes(upper, id_bits);
     const int hash = h0 > hu ? h0 : hu;
     return q + nr * 64 * 8 + 64 * 4 + 64 * 8 + 64 * 4 + nr * 64 * 4 + hash;
+}
+
+// Bytes of LDS one wave of the general register beam needs: query, new ids and row sums, the merge
+// / sort buffer (64 max(nr, nc) entries of key, id and flag), the radix histogram of the
+// candidate refill, the first spilled entry, and the visited hash.
+inline int lds_bytes_beam(int mode, int dim, int dimp, int h_log2, int id_bits, int nr, int nc) {
+    auto up = [](int v) { return (v + 15) & ~15; };
+    const int q = mode == kExactU8 ? up(dimp) : up(dim * 8);
+    const int b = 64 * (nr > nc ? nr : nc);
+    const int upper = h_log2 < 9 ? h_log2 : 9;
+    const int h0 = visited_hash_bytes(h_log2, id_bits), hu = visited_hash_bytes(upper, id_bits);
+    return q + 64 * 4 + 64 * 4 + b * 16 + 256 * 4 + 16 + (h0 > hu ? h0 : hu);
 }
 
 // Bytes of LDS one wave needs for the given capacities.
