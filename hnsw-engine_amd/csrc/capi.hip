@@ -162,8 +162,10 @@ bool two_phase_shape(const hm_index *h, int mode) {
 // Launches one batch. fixed == null: scratch from the handle's buffers; otherwise every scratch
 // array (and the counters / fail flags the batch leaves null) is carved from *fixed, the
 // bitsets and event counters cleared by a kernel first, and the wave slots reduced to what fits.
+// tier: 2 any kernel; 1 no kFast (a rerun after the register beam's tie list overflowed);
+// 0 the general kernel only.
 int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap, int max_slots,
-           hipStream_t s, bool allow_fast = true, Carver *fixed = nullptr) {
+           hipStream_t s, int tier = 2, Carver *fixed = nullptr) {
     Batch b = b0;
     SearchArgs a;
     memset(&a, 0, sizeof(a));
@@ -200,6 +202,7 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap,
     // (filters, codes, other shapes); beyond 512, or a rerun after a spill overflow, the general
     // kernel.
     const int nr_ef = beam_width(b.max_ef);
+    const bool allow_fast = tier >= 2;
     const bool fast = allow_fast && !filt && b.max_ef <= 64 && mode != kSeqU8;
     const int wide_hlog = h->tune_fast_hlog ? h->tune_fast_hlog : 12;
     const bool wide = allow_fast && !filt && !fast && !b.trace && mode == kExactU8 && nr_ef > 0 &&
@@ -207,8 +210,8 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap,
                       a.id_bits - wide_hlog <= 10;
     const bool fast_any = allow_fast && !filt && b.max_ef <= 64;
     const int kind = fast || wide || fast_any ? kFast
-                     : allow_fast && !b.trace && nr_ef > 0 ? kBeam
-                                                           : kGeneral;
+                     : tier >= 1 && !b.trace && nr_ef > 0 ? kBeam
+                                                          : kGeneral;
     plan(a, mode, filt, h, b.max_ef, kind, kind == kFast ? (wide ? nr_ef : 1) : nr_ef);
     if (fast && b.trace) {
         a.trace = b.trace;
@@ -238,7 +241,7 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap,
     if (fixed) {
         Carver &c = *fixed;
         int32_t *queue = c.take<int32_t>(1);
-        int32_t *stats = c.take<int32_t>(2);
+        int32_t *stats = c.take<int32_t>(4);
         if (!b.dc) b.dc = c.take<int32_t>(b.nq);
         if (!b.hops) b.hops = c.take<int32_t>(b.nq);
         if (!b.fail) b.fail = c.take<int32_t>(b.nq);
@@ -276,13 +279,13 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap,
         a.pre_hops = pre_hops;
         a.order = order;
         hipLaunchKernelGGL(clear_words_kernel, dim3(1), dim3(64), 0, s, (uint32_t *)stats,
-                           (int64_t)2);
+                           (int64_t)4);
         const int64_t words = (int64_t)slots * a.bitset_words;
         const int grid = (int)std::min<int64_t>(4096, (words + 1023) / 1024);
         hipLaunchKernelGGL(clear_words_kernel, dim3(std::max(1, grid)), dim3(256), 0, s,
                            a.bitsets, words);
     } else {
-        HM_TRY(h->stats.ensure(2, true, s), "stats allocation");
+        HM_TRY(h->stats.ensure(4, true, s), "stats allocation");
         a.stats = h->stats.p;
         HM_TRY(h->bits.ensure((size_t)slots * a.bitset_words, true, s), "bitset allocation");
         a.bitsets = h->bits.p;
@@ -453,33 +456,35 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
     b.dc = h->dc.p;
     b.hops = h->hops.p;
     b.fail = h->fail.p;
-    const int64_t spill = std::min<int64_t>(h->n, kDefaultSpill);
     // Queries that need different kernels (per-query ef; filtered and unfiltered ones) run as
     // one launch per kernel class, each over its own subset of the batch: class = (filtered,
     // register-beam width 1 / 2 / 4 / 8, or the general kernel beyond ef 512).
-    std::vector<int32_t> cls(nq);
-    int present = 0;
-    for (int64_t i = 0; i < nq; ++i) {
+    auto class_of = [&](int64_t i) {
         const bool fq = filt && (o->mask_index ? o->mask_index[i] >= 0 : o->mask_all >= 0);
         const int w = beam_width(efv.empty() ? b.ef_all : efv[i]);
-        cls[i] = (fq ? 5 : 0) + (w == 0 ? 4 : w == 1 ? 0 : w == 2 ? 1 : w == 4 ? 2 : 3);
-        present |= 1 << cls[i];
-    }
-    if (__builtin_popcount(present) <= 1) {
-        st = launch(h, mode, filt, b, spill, 0, s);
-        if (st) return st;
-    } else {
+        return (fq ? 5 : 0) + (w == 0 ? 4 : w == 1 ? 0 : w == 2 ? 1 : w == 4 ? 2 : 3);
+    };
+    // Runs the queries `which` (all of the batch when null), class by class; spill: candidate
+    // spill entries per wave slot, max_slots: a cap on wave slots (0: none).
+    auto run = [&](const std::vector<int32_t> *which, int64_t spill, int max_slots,
+                   int tier) -> int {
+        const int64_t count = which ? (int64_t)which->size() : nq;
         std::vector<int32_t> order;
-        order.reserve(nq);
+        order.reserve(count);
         std::vector<int64_t> start(11, 0);
+        int present = 0;
         for (int c = 0; c < 10; ++c) {
             start[c] = (int64_t)order.size();
-            for (int64_t i = 0; i < nq; ++i)
-                if (cls[i] == c) order.push_back((int32_t)i);
+            for (int64_t t = 0; t < count; ++t) {
+                const int64_t i = which ? (*which)[t] : t;
+                if (class_of(i) == c) order.push_back((int32_t)i);
+            }
+            if ((int64_t)order.size() > start[c]) present += 1;
         }
-        start[10] = nq;
-        HM_TRY(h->subset.ensure(nq), "alloc");
-        HM_TRY(hipMemcpyAsync(h->subset.p, order.data(), nq * 4, hipMemcpyHostToDevice, s),
+        start[10] = count;
+        if (!which && present == 1) return launch(h, mode, filt, b, spill, max_slots, s, tier);
+        HM_TRY(h->subset.ensure(count), "alloc");
+        HM_TRY(hipMemcpyAsync(h->subset.p, order.data(), count * 4, hipMemcpyHostToDevice, s),
                "copy subsets");
         for (int c = 0; c < 10; ++c) {
             const int64_t cnt = start[c + 1] - start[c];
@@ -491,34 +496,30 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
             for (int64_t t = start[c]; t < start[c + 1]; ++t)
                 mx = std::max(mx, efv.empty() ? b.ef_all : efv[order[t]]);
             bc.max_ef = mx;
-            st = launch(h, mode, filt && c >= 5, bc, spill, 0, s);
-            if (st) return st;
+            const int st2 = launch(h, mode, filt && c >= 5, bc, spill, max_slots, s, tier);
+            if (st2) return st2;
         }
-    }
+        HM_TRY(hipStreamSynchronize(s), "search");  // `order` must outlive its copy
+        return HM_OK;
+    };
+    st = run(nullptr, std::min<int64_t>(h->n, kDefaultSpill), 0, 2);
+    if (st) return st;
     std::vector<int32_t> failv(nq), dcv(nq), hv(nq);
     HM_TRY(hipMemcpyAsync(failv.data(), h->fail.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
     HM_TRY(hipStreamSynchronize(s), "search");
-    // Queries whose candidate spill overflowed run again with room for every node.
-    std::vector<int64_t> redo;
+    // Queries that failed (candidates beyond the spill, or evicted ties beyond the tie list of
+    // the kFast register beams) run again, as one batch per class, on the general register beam
+    // or the general kernel with room for every node (no query can hold more candidates than
+    // there are nodes), on as many wave slots as a spill budget of 4 GiB allows.
+    std::vector<int32_t> redo;
     for (int64_t i = 0; i < nq; ++i)
-        if (failv[i]) redo.push_back(i);
+        if (failv[i]) redo.push_back((int32_t)i);
     if (!redo.empty()) {
-        for (int64_t i : redo) {
-            Batch one = b;
-            one.nq = 1;
-            one.q32 = b.q32 ? b.q32 + (size_t)i * h->dim : nullptr;
-            one.q64 = b.q64 ? b.q64 + (size_t)i * h->dim : nullptr;
-            one.k = b.k ? b.k + i : nullptr;
-            one.ef = b.ef ? b.ef + i : nullptr;
-            one.midx = b.midx ? b.midx + i : nullptr;
-            one.ids = b.ids + (size_t)i * o->kstride;
-            one.dists = b.dists + (size_t)i * o->kstride;
-            one.dc = b.dc + i;
-            one.hops = b.hops + i;
-            one.fail = b.fail + i;
-            st = launch(h, mode, filt, one, h->n, 1, s, false);
-            if (st) return st;
-        }
+        h->reruns += (int64_t)redo.size();
+        const int64_t budget = int64_t(4) << 30;
+        const int slots = (int)std::max<int64_t>(1, std::min<int64_t>(1 << 20, budget / (h->n * 12)));
+        st = run(&redo, h->n, slots, 1);
+        if (st) return st;
         HM_TRY(hipMemcpyAsync(failv.data(), h->fail.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
         HM_TRY(hipStreamSynchronize(s), "search");
         for (int64_t i = 0; i < nq; ++i)
@@ -799,7 +800,7 @@ int hm_search_dev_noalloc(hm_index *h, const float *d_queries, const hm_search_o
     c.base = (uint8_t *)h->static_scratch;
     c.cap = kStaticScratchBytes;
     st = launch(h, exact_u8 ? kExactU8 : kSeqF32, filt, b, std::min<int64_t>(h->n, kDefaultSpill),
-                0, s, true, &c);
+                0, s, 2, &c);
     if (st) return st;
     HM_TRY(order_after(h, s), "stream order");
     return HM_OK;
@@ -984,11 +985,15 @@ int hm_set_tuning(hm_index *h, int32_t fast_hash_log2, int32_t hash_log2, int32_
 int hm_read_stats(hm_index *h, int32_t *out, int reset) {
     if (!h || !out) return invalid("null argument");
     HM_TRY(hipSetDevice(h->device), "hipSetDevice");
-    HM_TRY(h->stats.ensure(2, true, h->stream), "stats allocation");
+    HM_TRY(h->stats.ensure(4, true, h->stream), "stats allocation");
     HM_TRY(hipStreamSynchronize(h->stream), "sync");
     HM_TRY(hipDeviceSynchronize(), "sync");
-    HM_TRY(hipMemcpy(out, h->stats.p, 8, hipMemcpyDeviceToHost), "copy stats");
-    if (reset) HM_TRY(hipMemset(h->stats.p, 0, 8), "reset stats");
+    HM_TRY(hipMemcpy(out, h->stats.p, 12, hipMemcpyDeviceToHost), "copy stats");
+    out[3] = (int32_t)h->reruns;
+    if (reset) {
+        HM_TRY(hipMemset(h->stats.p, 0, 16), "reset stats");
+        h->reruns = 0;
+    }
     return HM_OK;
 }
 

@@ -53,6 +53,7 @@ struct hm_index {
     int tune_schedule = 0;  // 1: one launch per batch (no descents-first schedule)
     int tune_descent_cap = 0;  // > 0: descend_kernel visited entries per layer (else the maximum)
     hm::DevBuf<int32_t> stats;  // event counters, see SearchArgs::stats
+    int64_t reruns = 0;         // queries hm_search ran again after a spill or tie overflow
     // hm_search_graph_dev: the captured launches of the last batch shape and what they bake in
     hipGraphExec_t graph_exec = nullptr;
     std::vector<int64_t> graph_key;

@@ -1683,6 +1683,7 @@ HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
     }
     int spn = 0;          // spilled candidates
     E smin = E::inf();    // the smallest spilled one
+    bool spilled = false;
     auto worst_key = [&]() -> uint64_t {  // R's worst key when R is full, else +inf
         return rn == ef ? ent_slot<NR>(r, (ef - 1) >> 6).at((ef - 1) & 63).key() : kInfKey;
     };
@@ -1952,6 +1953,10 @@ HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                 const int keep = min(kC, min(cut_p, cut_s));
                 const int nspill = cut_p - keep;
                 if (nspill > 0 && (int64_t)spn + nspill > a.spill_cap) return false;
+                if (nspill > 0 && !spilled) {
+                    spilled = true;
+                    if (lane == 0 && a.stats) atomicAdd(&a.stats[2], 1);
+                }
                 // place every kept entry in the buffer and every spilled one in the spill; the
                 // entry at position keep is the smallest spilled this time
                 auto place = [&](const E &x, int p) {

@@ -92,7 +92,8 @@ struct SearchArgs {
     int32_t *trace_len;
     int32_t *trace_final;
     // event counters (atomic adds): [0] layer searches whose visited set moved from the LDS
-    // hash to the global bitset, [1] queries whose evicted ties outgrew the register beam
+    // hash to the global bitset, [1] queries whose evicted ties outgrew the register beam,
+    // [2] layer searches of the general register beam whose candidates spilled to global memory
     int32_t *stats;
 };
 

@@ -272,9 +272,12 @@ int hm_set_tuning(hm_index *index, int32_t fast_hash_log2, int32_t hash_log2,
                   int32_t waves_per_cu, int32_t probe_limit, int32_t one_launch,
                   int32_t descent_cap);
 
-/* Event counters since the last reset: out[0] layer searches whose visited set outgrew the LDS
- * hash, out[1] queries whose evicted ties outgrew the register beam (re-run on the general
- * kernel by hm_search). Synchronises the device. */
+/* Event counters since the last reset (out: 4 int32): out[0] layer searches whose visited set
+ * outgrew the LDS hash, out[1] queries whose evicted ties outgrew the fixed tie list of the
+ * register beam, out[2] layer searches of the general register beam whose candidates spilled
+ * to global memory, out[3] queries hm_search ran again (after a tie overflow, or candidates
+ * beyond the spill) on the general register beam with room for every node. Synchronises the
+ * device. */
 int hm_read_stats(hm_index *index, int32_t *out, int reset);
 
 const char *hm_last_error(void);

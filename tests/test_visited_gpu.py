@@ -33,7 +33,7 @@ def _tune(eng, fast_hash_log2=0, probe_limit=0, waves_per_cu=0, one_launch=0,
 
 def _moves(eng) -> int:
     """Layer searches whose visited set moved to the global bitset since the last call."""
-    out = (ctypes.c_int32 * 2)()
+    out = (ctypes.c_int32 * 4)()
     _native.check(_native.lib().hm_read_stats(eng._device(), out, 1))
     return int(out[0])
 
