@@ -962,7 +962,7 @@ int hm_search_codes_rerank(hm_index *h, const double *code_queries, const float 
         return invalid("k must be in [1, fetch] and the traversal must keep fetch candidates");
     int p2 = 1;
     while (p2 < o->kstride) p2 <<= 1;
-    if (p2 * 12 + 16 + h->dim * 8 > 160 * 1024) return invalid("rerank fetch too large");
+    if (p2 * 12 > 160 * 1024) return invalid("rerank fetch too large");
     return search_host(h, kSeqU8, nullptr, code_queries, o, ids, dists, dcount, hops, queries, k);
 }
 

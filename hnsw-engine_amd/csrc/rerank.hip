@@ -9,24 +9,10 @@
 #include "../../include/hnsw_mi355x.h"
 #include "index.hpp"
 #include "keys.hpp"
+#include "seqdist.hpp"
 #include "wave.hpp"
 
 namespace hm {
-
-// numba_engine.py:57-70 in its exact order (float64, left to right, unfused).
-HM_DEV double seq_distance(const float *row, const double *q, int dim, int metric) {
-#pragma clang fp contract(off)
-    double acc = 0.0;
-    if (metric == kL2) {
-        for (int i = 0; i < dim; ++i) {
-            double d = (double)row[i] - q[i];
-            acc = acc + d * d;
-        }
-        return acc;
-    }
-    for (int i = 0; i < dim; ++i) acc = acc + (double)row[i] * q[i];
-    return metric == kIP ? -acc : 1.0 - acc;
-}
 
 __global__ __launch_bounds__(64) void rerank_kernel(const float *vec, int dim, int metric,
                                                     const float *queries, const int64_t *cand,
@@ -35,11 +21,8 @@ __global__ __launch_bounds__(64) void rerank_kernel(const float *vec, int dim, i
     extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
     uint64_t *keys = (uint64_t *)lds;
     uint32_t *ids = (uint32_t *)(lds + (size_t)p2 * 8);
-    double *q = (double *)(lds + (size_t)p2 * 12 + ((16 - (p2 * 12) % 16) % 16));
     const int lane = lane_id();
     const int qi = blockIdx.x;
-    for (int i = lane; i < dim; i += 64) q[i] = (double)queries[(size_t)qi * dim + i];
-    wave_sync();
     int valid = 0;
     for (int j = lane; j < fetch; j += 64) {
         const int64_t e = cand[(size_t)qi * fetch + j];
@@ -55,7 +38,8 @@ __global__ __launch_bounds__(64) void rerank_kernel(const float *vec, int dim, i
         if (e >= 0) {
             const int d = w + rank_below(m);
             ids[d] = (uint32_t)e;
-            keys[d] = key_of_double(seq_distance(vec + (size_t)e * dim, q, dim, metric));
+            keys[d] = key_of_double(
+                seq_row<false>(vec + (size_t)e * dim, queries + (size_t)qi * dim, dim, metric));
         }
         w += popc(m);
     }
@@ -77,7 +61,7 @@ hipError_t rerank_device(hm_index *h, const float *d_q, const int64_t *d_cand, i
                          int fetch, int k, int64_t *d_ids, double *d_d, hipStream_t s) {
     int p2 = 1;
     while (p2 < fetch) p2 <<= 1;
-    const int lds = p2 * 12 + 16 + h->dim * 8;
+    const int lds = p2 * 12;
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     if (lds > 64 * 1024) {
         const hipError_t e = hipFuncSetAttribute((const void *)rerank_kernel,
@@ -103,7 +87,7 @@ extern "C" int hm_rerank(hm_index *h, const float *queries, int64_t nq, const in
             return invalid("candidate ids outside [-1, n)");
     int p2 = 1;
     while (p2 < fetch) p2 <<= 1;
-    if (p2 * 12 + 16 + h->dim * 8 > 160 * 1024) return invalid("rerank fetch too large");
+    if (p2 * 12 > 160 * 1024) return invalid("rerank fetch too large");
     hipStream_t s = h->stream;
     HM_TRY(hipSetDevice(h->device), "hipSetDevice");
     HM_TRY(h->q32.ensure((size_t)nq * h->dim), "alloc");

@@ -19,9 +19,12 @@
 // refill), a visited hash (overflow: a per-wave global bitset), staging for one expansion.
 // Candidates with d > worst(R) once R is full can never be expanded (the first such pop ends the
 // search), so they are dropped at merge time; d == worst is kept.
+#include <utility>
+
 #include "search.hpp"
 #include "wave.hpp"
 #include "keys.hpp"
+#include "seqdist.hpp"
 
 namespace hm {
 
@@ -43,9 +46,10 @@ struct Lds {
     uint32_t *model;  // learned stop: the packed model, kModelWords words per lane
     uint32_t *hist;  // 256 bins for the refill's radix select
     uint64_t *misc;  // [0..1]: first spilled entry
-    // general register beam: merge / sort buffer of 64 max(nr, nc) entries
-    uint64_t *bufk;
-    uint32_t *bufi, *buff;
+    // general register beam: R (64 nr entries: key, id, expanded flag) and C (64 nc entries)
+    // as the merges leave them, sorted
+    uint64_t *bufk, *ck_b;
+    uint32_t *bufi, *buff, *ci_b;
 };
 
 HM_DEV Lds carve(uint8_t *base, const SearchArgs &a, int mode) {
@@ -119,10 +123,11 @@ HM_DEV Lds carve_beam(uint8_t *base, const SearchArgs &a, int mode) {
     p += mode == kExactU8 ? up(a.dimp) : up(a.dim * 8);
     L.nbi = (uint32_t *)p; p += 64 * 4;
     L.nbacc = (int32_t *)p; p += 64 * 4;
-    const int b = 64 * (a.nr > a.nc ? a.nr : a.nc);
-    L.bufk = (uint64_t *)p; p += b * 8;
-    L.bufi = (uint32_t *)p; p += b * 4;
-    L.buff = (uint32_t *)p; p += b * 4;
+    L.bufk = (uint64_t *)p; p += 64 * a.nr * 8;
+    L.bufi = (uint32_t *)p; p += 64 * a.nr * 4;
+    L.buff = (uint32_t *)p; p += 64 * a.nr * 4;
+    L.ck_b = (uint64_t *)p; p += 64 * a.nc * 8;
+    L.ci_b = (uint32_t *)p; p += 64 * a.nc * 4;
     L.hist = (uint32_t *)p; p += 256 * 4;
     L.misc = (uint64_t *)p; p += 16;
     L.hash = (uint32_t *)p;
@@ -339,74 +344,25 @@ HM_DEV uint64_t dist_exact(const SearchArgs &a, const Lds &L, int nnew, int32_t 
 }
 
 // numba_engine.py:57-70 in its exact operation order: one lane per neighbour, float64,
-// left to right, no fused multiply-add.
+// left to right, no fused multiply-add (seq_row).
 template <bool U8>
-HM_DEV uint64_t dist_seq(const SearchArgs &a, const Lds &L, int nnew) {
-#pragma clang fp contract(off)
+HM_DEV uint64_t dist_seq(const SearchArgs &a, const Lds &L, int nnew, int qi) {
     const int lane = lane_id();
     if (lane >= nnew) return kInfKey;
-    const double *q = (const double *)L.qb;
     const uint32_t id = L.nbi[lane];
-    const int dim = a.dim;
-    double acc = 0.0;
-    if (!U8) {
-        const float *row = a.vec_f32 + (size_t)id * dim;
-        if ((dim & 3) == 0) {
-            const float4 *r4 = (const float4 *)row;
-            if (a.metric == kL2) {
-                for (int i = 0; i < dim; i += 4) {
-                    float4 x = r4[i >> 2];
-                    double d0 = (double)x.x - q[i];
-                    acc = acc + d0 * d0;
-                    double d1 = (double)x.y - q[i + 1];
-                    acc = acc + d1 * d1;
-                    double d2 = (double)x.z - q[i + 2];
-                    acc = acc + d2 * d2;
-                    double d3 = (double)x.w - q[i + 3];
-                    acc = acc + d3 * d3;
-                }
-            } else {
-                for (int i = 0; i < dim; i += 4) {
-                    float4 x = r4[i >> 2];
-                    acc = acc + (double)x.x * q[i];
-                    acc = acc + (double)x.y * q[i + 1];
-                    acc = acc + (double)x.z * q[i + 2];
-                    acc = acc + (double)x.w * q[i + 3];
-                }
-            }
-        } else if (a.metric == kL2) {
-            for (int i = 0; i < dim; ++i) {
-                double d = (double)row[i] - q[i];
-                acc = acc + d * d;
-            }
-        } else {
-            for (int i = 0; i < dim; ++i) acc = acc + (double)row[i] * q[i];
-        }
-    } else {
-        const uint8_t *row = a.vec_u8 + (size_t)id * a.dimp;
-        for (int i0 = 0; i0 < dim; i0 += 16) {
-            uint4 w = *(const uint4 *)(row + i0);
-            uint32_t ws[4] = {w.x, w.y, w.z, w.w};
-            const int lim = dim - i0 < 16 ? dim - i0 : 16;
-            for (int i = 0; i < lim; ++i) {
-                double x = (double)((ws[i >> 2] >> (8 * (i & 3))) & 255u);
-                if (a.metric == kL2) {
-                    double d = x - q[i0 + i];
-                    acc = acc + d * d;
-                } else {
-                    acc = acc + x * q[i0 + i];
-                }
-            }
-        }
-    }
-    double d = a.metric == kL2 ? acc : (a.metric == kIP ? -acc : 1.0 - acc);
-    return key_of_double(d);
+    if constexpr (U8)
+        return key_of_double(seq_row<true>(a.vec_u8 + (size_t)id * a.dimp,
+                                           a.q_f64 + (size_t)qi * a.dim, a.dim, a.metric));
+    else
+        return key_of_double(seq_row<false>(a.vec_f32 + (size_t)id * a.dim,
+                                            a.q_f32 + (size_t)qi * a.dim, a.dim, a.metric));
 }
 
+// Distances of the nnew new ids to query qi (lanes 0..nnew-1 get their key).
 template <int MODE, int SH = kShapeAny>
-HM_DEV uint64_t distances(const SearchArgs &a, const Lds &L, int nnew, int32_t qq) {
+HM_DEV uint64_t distances(const SearchArgs &a, const Lds &L, int nnew, int32_t qq, int qi) {
     if constexpr (MODE == kExactU8) return dist_exact<SH>(a, L, nnew, qq);
-    else return dist_seq<MODE == kSeqU8>(a, L, nnew);
+    else return dist_seq<MODE == kSeqU8>(a, L, nnew, qi);
 }
 
 template <int MODE>
@@ -606,6 +562,7 @@ HM_DEV bool visit_lanes_quotient(const Lds &L, Visited &V, uint32_t id, bool act
 // ---------------------------------------------------------------------------- query state
 
 struct Query {
+    int qi;  // index of the query in the batch
     int ef, k;
     const uint32_t *mask;  // null: every node allowed
     int32_t qq;
@@ -648,7 +605,7 @@ HM_DEV void greedy_layer(const SearchArgs &a, const Lds &L, Visited &V, Query &Q
             wave_sync();
             V.count += nnew;
             const int32_t nuo = lane < nnew ? a.upper_off[L.nbi[lane]] : 0;
-            uint64_t key = distances<MODE, SH>(a, L, nnew, Q.qq);
+            uint64_t key = distances<MODE, SH>(a, L, nnew, Q.qq, Q.qi);
             Q.dcount += nnew;
             if constexpr (MODE == kExactU8) {
                 // Exact keys fit in 32 bits: the row minimum by DPP, its first lane by ballot.
@@ -861,7 +818,7 @@ HM_DEV bool layer0(const SearchArgs &a, const Lds &L, Visited &V, Query &Q, int 
             if (isnew) L.nbi[rank_below(nm)] = (uint32_t)nb;
             wave_sync();
             V.count += nnew;
-            const uint64_t ki = distances<MODE>(a, L, nnew, Q.qq);
+            const uint64_t ki = distances<MODE>(a, L, nnew, Q.qq, Q.qi);
             Q.dcount += nnew;
             const uint32_t ii = lane < nnew ? L.nbi[lane] : kInfId;
             const bool al = FILT && lane < nnew ? allowed(Q, ii) : true;
@@ -1167,7 +1124,7 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
             if (isnew) L.nbi[rank_below(nm)] = (uint32_t)nb;
             wave_sync();
             V.count += nnew;
-            const uint64_t ki = distances<MODE, SH>(a, L, nnew, Q.qq);
+            const uint64_t ki = distances<MODE, SH>(a, L, nnew, Q.qq, Q.qi);
             const uint32_t ii = lane < nnew ? L.nbi[lane] : kInfId;
             Q.dcount += nnew;
             if constexpr (AUX != 0) {
@@ -1392,7 +1349,7 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
             if (isnew) L.nbi[rank_below(nm)] = (uint32_t)nb;
             wave_sync();
             V.count += nnew;
-            const uint64_t ki = distances<MODE, kShapeM16D128>(a, L, nnew, Q.qq);
+            const uint64_t ki = distances<MODE, kShapeM16D128>(a, L, nnew, Q.qq, Q.qi);
             const uint64_t ne = lane < nnew ? (ki << 32) | L.nbi[lane] : kInfEntry;
             const uint32_t nk = key_hi(ne);
             Q.dcount += nnew;
@@ -1594,8 +1551,8 @@ template <> struct Ent<true> {
     HM_DEV uint32_t id() const { return (uint32_t)w; }
     HM_DEV bool lt(const Ent &o) const { return w < o.w; }
     HM_DEV Ent at(int l) const { return Ent{readlane_u64(w, l)}; }  // lane l's entry, uniform
-    HM_DEV void put(const Lds &L, int p) const { L.bufk[p] = w; }
-    static HM_DEV Ent get(const Lds &L, int p) { return Ent{L.bufk[p]}; }
+    HM_DEV void put(uint64_t *K, uint32_t *, int p) const { K[p] = w; }
+    static HM_DEV Ent get(const uint64_t *K, const uint32_t *, int p) { return Ent{K[p]}; }
 };
 
 template <> struct Ent<false> {
@@ -1607,12 +1564,22 @@ template <> struct Ent<false> {
     HM_DEV uint32_t id() const { return i; }
     HM_DEV bool lt(const Ent &o) const { return k < o.k || (k == o.k && i < o.i); }
     HM_DEV Ent at(int l) const { return Ent{readlane_u64(k, l), readlane_u32(i, l)}; }
-    HM_DEV void put(const Lds &L, int p) const {
-        L.bufk[p] = k;
-        L.bufi[p] = i;
+    HM_DEV void put(uint64_t *K, uint32_t *I, int p) const {
+        K[p] = k;
+        I[p] = i;
     }
-    static HM_DEV Ent get(const Lds &L, int p) { return Ent{L.bufk[p], L.bufi[p]}; }
+    static HM_DEV Ent get(const uint64_t *K, const uint32_t *I, int p) { return Ent{K[p], I[p]}; }
 };
+
+// f(std::integral_constant<int, i>) for i = 0..N-1: slot indices known at compile time, so
+// register arrays indexed by them stay in registers.
+template <class F, int... I>
+HM_DEV void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F> HM_DEV void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 template <int N, class E> HM_DEV E ent_slot(const E (&v)[N], int s) {
     E r = v[0];
@@ -1674,37 +1641,55 @@ HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
     {
         const E e0 = E::make(entry_k, entry_id);
         if (!FILT || allowed(Q, entry_id)) {
-            if (lane == 0) r[0] = e0;
+            if (lane == 0) {
+                r[0] = e0;
+                e0.put(L.bufk, L.bufi, 0);
+            }
             rn = 1;
         } else {
-            if (lane == 0) c[0] = e0;
+            if (lane == 0) {
+                c[0] = e0;
+                e0.put(L.ck_b, L.ci_b, 0);
+            }
             cn = 1;
         }
+        wave_sync();
     }
     int spn = 0;          // spilled candidates
     E smin = E::inf();    // the smallest spilled one
     bool spilled = false;
+    // R and C are also kept, sorted, in LDS: entries picked by a wave-uniform index come from
+    // there (picking them from the register arrays would index those at run time)
     auto worst_key = [&]() -> uint64_t {  // R's worst key when R is full, else +inf
-        return rn == ef ? ent_slot<NR>(r, (ef - 1) >> 6).at((ef - 1) & 63).key() : kInfKey;
+        return rn == ef ? E::get(L.bufk, L.bufi, ef - 1).key() : kInfKey;
     };
     for (;;) {
         if (ch == cn && spn > 0) {
             // C ran dry: prune the spill, then bring back its smallest kC entries.
             __threadfence();
             const uint64_t wk = worst_key();
+            // in place: the kept entries move down, never past one not yet read (4 x 64
+            // entries are read before any of them is written)
             int w = 0;
-            for (int b = 0; b < spn; b += 64) {
-                const int j = b + lane;
-                const bool v = j < spn;
-                const uint64_t k = v ? sk[j] : kInfKey;
-                const uint32_t id = v ? si[j] : kInfId;
-                const bool kp = v && k <= wk;
-                const uint64_t m = ballot(kp);
-                if (kp) {
-                    sk[w + rank_below(m)] = k;
-                    si[w + rank_below(m)] = id;
+            for (int b = 0; b < spn; b += 256) {
+                uint64_t k4[4];
+                uint32_t i4[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int j = b + 64 * t + lane;
+                    k4[t] = j < spn ? sk[j] : kInfKey;
+                    i4[t] = j < spn ? si[j] : kInfId;
                 }
-                w += popc(m);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const bool kp = b + 64 * t + lane < spn && k4[t] <= wk;
+                    const uint64_t m = ballot(kp);
+                    if (kp) {
+                        sk[w + rank_below(m)] = k4[t];
+                        si[w + rank_below(m)] = i4[t];
+                    }
+                    w += popc(m);
+                }
             }
             __threadfence();
             spn = 0;
@@ -1717,28 +1702,38 @@ HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                 int nc = 0, nw = 0;
                 uint64_t mk = kInfKey;
                 uint32_t mi = kInfId;
-                for (int b = 0; b < w; b += 64) {
-                    const int j = b + lane;
-                    const bool v = j < w;
-                    const uint64_t k = v ? sk[j] : kInfKey;
-                    const uint32_t id = v ? si[j] : kInfId;
-                    const bool sel = v && less(k, id, bk, bi);
-                    const bool rest = v && !sel;
-                    const uint64_t m1 = ballot(sel), m2 = ballot(rest);
-                    if (sel) {
-                        L.bufk[nc + rank_below(m1)] = k;
-                        L.bufi[nc + rank_below(m1)] = id;
+                for (int b = 0; b < w; b += 256) {
+                    uint64_t k4[4];
+                    uint32_t i4[4];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const int j = b + 64 * t + lane;
+                        k4[t] = j < w ? sk[j] : kInfKey;
+                        i4[t] = j < w ? si[j] : kInfId;
                     }
-                    if (rest) {
-                        sk[nw + rank_below(m2)] = k;
-                        si[nw + rank_below(m2)] = id;
-                        if (less(k, id, mk, mi)) {
-                            mk = k;
-                            mi = id;
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const bool v = b + 64 * t + lane < w;
+                        const uint64_t k = k4[t];
+                        const uint32_t id = i4[t];
+                        const bool sel = v && less(k, id, bk, bi);
+                        const bool rest = v && !sel;
+                        const uint64_t m1 = ballot(sel), m2 = ballot(rest);
+                        if (sel) {
+                            L.ck_b[nc + rank_below(m1)] = k;
+                            L.ci_b[nc + rank_below(m1)] = id;
                         }
+                        if (rest) {
+                            sk[nw + rank_below(m2)] = k;
+                            si[nw + rank_below(m2)] = id;
+                            if (less(k, id, mk, mi)) {
+                                mk = k;
+                                mi = id;
+                            }
+                        }
+                        nc += popc(m1);
+                        nw += popc(m2);
                     }
-                    nc += popc(m1);
-                    nw += popc(m2);
                 }
                 __threadfence();
                 if (nc == 0) return false;  // cannot happen: the select takes 1..kC entries
@@ -1747,29 +1742,30 @@ HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                 spn = nw;
                 int p2 = 1;
                 while (p2 < nc) p2 <<= 1;
-                bitonic_sort(L.bufk, L.bufi, nc, p2);
+                bitonic_sort(L.ck_b, L.ci_b, nc, p2);
 #pragma unroll
                 for (int s = 0; s < NC; ++s) {
                     const int e = lane + 64 * s;
-                    c[s] = e < nc ? E::make(L.bufk[e], L.bufi[e]) : E::inf();
+                    c[s] = e < nc ? E::make(L.ck_b[e], L.ci_b[e]) : E::inf();
+                    if (e < nc) c[s].put(L.ck_b, L.ci_b, e);  // packed form for the head reads
                 }
                 cn = nc;
                 wave_sync();
             }
         }
         // The next candidate: R's first unexpanded entry or C's, whichever is smaller.
-        E er = E::inf();
-        int rs = -1, rp = 0;
+        // (the slot is picked by selects, not by indexing, so that R stays in registers)
+        int rs = -1;
+        uint64_t rlive = 0;
 #pragma unroll
-        for (int s = 0; s < NR; ++s) {
+        for (int s = NR - 1; s >= 0; --s) {
             const uint64_t live = slot_mask(rn, s) & ~rexp[s];
-            if (rs < 0 && live) {
-                rp = __ffsll((long long)live) - 1;
-                er = r[s].at(rp);
-                rs = s;
-            }
+            rs = live ? s : rs;
+            rlive = live ? live : rlive;
         }
-        const E ec = ch < cn ? ent_slot<NC>(c, ch >> 6).at(ch & 63) : E::inf();
+        const int rp = rs >= 0 ? __ffsll((long long)rlive) - 1 : 0;
+        const E er = rs >= 0 ? E::get(L.bufk, L.bufi, 64 * rs + rp) : E::inf();
+        const E ec = ch < cn ? E::get(L.ck_b, L.ci_b, ch) : E::inf();
         if (rs < 0 && ch >= cn) break;
         E cur;
         if (rs >= 0 && (ch >= cn || er.lt(ec))) {
@@ -1797,7 +1793,7 @@ HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
             if (isnew) L.nbi[rank_below(nm)] = (uint32_t)nb;
             wave_sync();
             V.count += nnew;
-            const uint64_t ki = distances<MODE, SH>(a, L, nnew, Q.qq);
+            const uint64_t ki = distances<MODE, SH>(a, L, nnew, Q.qq, Q.qi);
             const uint32_t ii = lane < nnew ? L.nbi[lane] : kInfId;
             Q.dcount += nnew;
             const E ne = lane < nnew ? E::make(ki, ii) : E::inf();
@@ -1849,12 +1845,12 @@ HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
 #pragma unroll
                 for (int s = 0; s < NR; ++s) {
                     if (((rvalid[s] >> lane) & 1ull) && pos_r[s] < ef) {
-                        r[s].put(L, pos_r[s]);
+                        r[s].put(L.bufk, L.bufi, pos_r[s]);
                         L.buff[pos_r[s]] = (uint32_t)((rexp[s] >> lane) & 1ull);
                     }
                 }
                 if (acc_al && pos_a < ef) {
-                    ne.put(L, pos_a);
+                    ne.put(L.bufk, L.bufi, pos_a);
                     L.buff[pos_a] = 0u;
                 }
                 wave_sync();
@@ -1862,7 +1858,7 @@ HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                 for (int s = 0; s < NR; ++s) {
                     const int e = lane + 64 * s;
                     const bool in = e < nrn;
-                    nr[s] = in ? E::get(L, e) : E::inf();
+                    nr[s] = in ? E::get(L.bufk, L.bufi, e) : E::inf();
                     nexp[s] = ballot(in && L.buff[e] != 0u);
                     ev_r[s] = ((rvalid[s] >> lane) & 1ull) && pos_r[s] >= ef &&
                               !((rexp[s] >> lane) & 1ull);
@@ -1880,15 +1876,15 @@ HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
             // unexpanded entries pushed out of R (the new ones among them included).
             const bool s_new = (acc && !acc_al) || (acc_al && pos_a >= ef);
             uint64_t sm_r[NR];
-            int ns = popc(ballot(s_new));
+            int ns = popc(ballot(s_new)), nev = 0;
 #pragma unroll
             for (int s = 0; s < NR; ++s) {
                 sm_r[s] = ballot(ev_r[s]);
-                ns += popc(sm_r[s]);
+                nev += popc(sm_r[s]);
             }
+            ns += nev;
             if (ns > 0) {
-                const uint64_t nwk =
-                    nrn == ef ? ent_slot<NR>(nr, (ef - 1) >> 6).at((ef - 1) & 63).key() : kInfKey;
+                const uint64_t nwk = nrn == ef ? E::get(L.bufk, L.bufi, ef - 1).key() : kInfKey;
                 uint64_t clive[NC];
 #pragma unroll
                 for (int s = 0; s < NC; ++s) clive[s] = slot_mask(cn, s) & ~slot_mask(ch, s);
@@ -1915,20 +1911,24 @@ HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                     for (int s = 0; s < NR; ++s)
                         if (holder == s && lane == q) hb_r[s] = hb;
                     if (x.lt(ne)) xb_new += 1;
+                    if (nev > 0) {  // entries pushed out of R are new candidates too
 #pragma unroll
-                    for (int s = 0; s < NR; ++s)
-                        if (x.lt(r[s])) xb_r[s] += 1;
+                        for (int s = 0; s < NR; ++s)
+                            if (x.lt(r[s])) xb_r[s] += 1;
+                    }
                 };
                 for (uint64_t m = ballot(s_new); m; m &= m - 1ull) {
                     const int q = __ffsll((long long)m) - 1;
                     step(ne.at(q), q, -1);
                 }
-#pragma unroll
-                for (int s = 0; s < NR; ++s)
-                    for (uint64_t m = sm_r[s]; m; m &= m - 1ull) {
-                        const int q = __ffsll((long long)m) - 1;
-                        step(r[s].at(q), q, s);
-                    }
+                if (nev > 0)
+                    static_for<NR>([&](auto si) {
+                        constexpr int s = decltype(si)::value;
+                        for (uint64_t m = sm_r[s]; m; m &= m - 1ull) {
+                            const int q = __ffsll((long long)m) - 1;
+                            step(r[s].at(q), q, s);
+                        }
+                    });
                 const int total = cn - ch + ns;
                 // sorted, so what lies beyond the new worst key (R full) is a suffix, and so is
                 // what is not below the smallest spilled entry
@@ -1942,10 +1942,12 @@ HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                     }
                     np += popc(ballot(s_new && ne.key() <= nwk));
                     nsp += popc(ballot(s_new && ne.lt(smin)));
+                    if (nev > 0) {
 #pragma unroll
-                    for (int s = 0; s < NR; ++s) {
-                        np += popc(ballot(r[s].key() <= nwk) & sm_r[s]);
-                        nsp += popc(ballot(r[s].lt(smin)) & sm_r[s]);
+                        for (int s = 0; s < NR; ++s) {
+                            np += popc(ballot(r[s].key() <= nwk) & sm_r[s]);
+                            nsp += popc(ballot(r[s].lt(smin)) & sm_r[s]);
+                        }
                     }
                     if (nrn == ef) cut_p = np;
                     if (spn > 0) cut_s = nsp;
@@ -1961,7 +1963,7 @@ HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                 // entry at position keep is the smallest spilled this time
                 auto place = [&](const E &x, int p) {
                     if (p < keep) {
-                        x.put(L, p);
+                        x.put(L.ck_b, L.ci_b, p);
                     } else if (p < cut_p) {
                         sk[spn + p - keep] = x.key();
                         si[spn + p - keep] = x.id();
@@ -1975,14 +1977,16 @@ HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                 for (int s = 0; s < NC; ++s)
                     if ((clive[s] >> lane) & 1ull) place(c[s], pc[s]);
                 if (s_new) place(ne, hb_new + xb_new);
+                if (nev > 0) {
 #pragma unroll
-                for (int s = 0; s < NR; ++s)
-                    if ((sm_r[s] >> lane) & 1ull) place(r[s], hb_r[s] + xb_r[s]);
+                    for (int s = 0; s < NR; ++s)
+                        if ((sm_r[s] >> lane) & 1ull) place(r[s], hb_r[s] + xb_r[s]);
+                }
                 wave_sync();
 #pragma unroll
                 for (int s = 0; s < NC; ++s) {
                     const int e = lane + 64 * s;
-                    c[s] = e < keep ? E::get(L, e) : E::inf();
+                    c[s] = e < keep ? E::get(L.ck_b, L.ci_b, e) : E::inf();
                 }
                 if (nspill > 0) {
                     const E first = E::make(L.misc[0], (uint32_t)L.misc[1]);
@@ -2019,6 +2023,7 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
                       const StopModel &M) {
     const int lane = lane_id();
     Query Q;
+    Q.qi = qi;
     Q.ef = a.ef_q ? a.ef_q[qi] : a.ef_all;
     Q.k = a.k_q ? a.k_q[qi] : a.k_all;
     Q.mask = nullptr;
@@ -2058,7 +2063,7 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
         if (lane == 0) L.nbi[0] = (uint32_t)a.entry;
         wave_sync();
         // Entry point distance (numba_engine.py:449-451), then greedy descent.
-        cur_k = shfl_u64(distances<MODE, SH>(a, L, 1, Q.qq), 0);
+        cur_k = shfl_u64(distances<MODE, SH>(a, L, 1, Q.qq, Q.qi), 0);
         cur_id = (uint32_t)a.entry;
         Q.dcount = 1;
         wave_sync();

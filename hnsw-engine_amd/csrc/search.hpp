@@ -136,16 +136,15 @@ inline int lds_bytes_fast(int mode, int dim, int dimp, int h_log2, int id_bits, 
     return q + nr * 64 * 8 + 64 * 4 + 64 * 8 + 64 * 4 + nr * 64 * 4 + hash;
 }
 
-// Bytes of LDS one wave of the general register beam needs: query, new ids and row sums, the merge
-// / sort buffer (64 max(nr, nc) entries of key, id and flag), the radix histogram of the
-// candidate refill, the first spilled entry, and the visited hash.
+// Bytes of LDS one wave of the general register beam needs: query, new ids and row sums, R
+// (64 nr entries of key, id and expanded flag), C (64 nc entries of key and id), the radix
+// histogram of the candidate refill, the first spilled entry, and the visited hash.
 inline int lds_bytes_beam(int mode, int dim, int dimp, int h_log2, int id_bits, int nr, int nc) {
     auto up = [](int v) { return (v + 15) & ~15; };
     const int q = mode == kExactU8 ? up(dimp) : up(dim * 8);
-    const int b = 64 * (nr > nc ? nr : nc);
     const int upper = h_log2 < 9 ? h_log2 : 9;
     const int h0 = visited_hash_bytes(h_log2, id_bits), hu = visited_hash_bytes(upper, id_bits);
-    return q + 64 * 4 + 64 * 4 + b * 16 + 256 * 4 + 16 + (h0 > hu ? h0 : hu);
+    return q + 64 * 4 + 64 * 4 + 64 * nr * 16 + 64 * nc * 12 + 256 * 4 + 16 + (h0 > hu ? h0 : hu);
 }
 
 // Bytes of LDS one wave needs for the given capacities.

@@ -1,0 +1,86 @@
+// The reference's float64 distance (numba_engine.py:57-70) of one row per lane: float64, left
+// to right, no fused multiply-add: l2 = sum (x_i - q_i)^2, ip = -sum x_i q_i, cosine =
+// 1 - sum x_i q_i. The row is read in 16-byte chunks, four at a time, and the next four are in
+// flight while the current ones are summed, so a row costs about one memory latency instead of
+// one per chunk. The query (dim float64 or float32 values in global memory, the same for every
+// lane) is read through the scalar cache into scalar registers, off the chain of dependent
+// adds; a float32 query value converts to float64 exactly, as the reference's does.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "search.hpp"
+#include "wave.hpp"
+
+namespace hm {
+
+// Adds the elements of one 16-byte chunk (16 bytes of U8 codes or 4 floats), element i0 first,
+// the ones at dim and beyond skipped.
+// q[i] as float64, by a scalar load (the address is wave-uniform and the data read-only).
+template <class QT> HM_DEV double qval(const QT *q, int i) {
+    using CP = const __attribute__((address_space(4))) QT *;
+    return (double)((CP)q)[i];
+}
+
+template <bool U8, class QT>
+HM_DEV void seq_chunk(double &acc, uint4 w, const QT *q, int i0, int dim, bool l2) {
+#pragma clang fp contract(off)
+    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+    constexpr int kPer = U8 ? 16 : 4;
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int i = i0 + e;
+        if (i >= dim) break;
+        const double x = U8 ? (double)((ws[e >> 2] >> (8 * (e & 3))) & 255u)
+                            : (double)__uint_as_float(ws[e]);
+        const double qi = qval(q, i);
+        if (l2) {
+            const double d = x - qi;
+            acc = acc + d * d;
+        } else {
+            acc = acc + x * qi;
+        }
+    }
+}
+
+template <bool U8, class QT>
+HM_DEV double seq_row(const void *row, const QT *q, int dim, int metric) {
+#pragma clang fp contract(off)
+    const bool l2 = metric == kL2;
+    double acc = 0.0;
+    if (!U8 && (dim & 3) != 0) {  // float rows not 16-byte aligned: element by element
+        const float *r = (const float *)row;
+        for (int i = 0; i < dim; ++i) {
+            const double qi = qval(q, i);
+            if (l2) {
+                const double d = (double)r[i] - qi;
+                acc = acc + d * d;
+            } else {
+                acc = acc + (double)r[i] * qi;
+            }
+        }
+    } else {
+        constexpr int kPer = U8 ? 16 : 4;
+        constexpr int kB = 4;
+        const uint4 *r4 = (const uint4 *)row;
+        const int nch = (dim + kPer - 1) / kPer;
+        uint4 cur[kB];
+#pragma unroll
+        for (int t = 0; t < kB; ++t)
+            if (t < nch) cur[t] = r4[t];
+        for (int b0 = 0; b0 < nch; b0 += kB) {
+            uint4 nxt[kB];
+#pragma unroll
+            for (int t = 0; t < kB; ++t)
+                if (b0 + kB + t < nch) nxt[t] = r4[b0 + kB + t];
+#pragma unroll
+            for (int t = 0; t < kB; ++t)
+                if (b0 + t < nch) seq_chunk<U8, QT>(acc, cur[t], q, (b0 + t) * kPer, dim, l2);
+#pragma unroll
+            for (int t = 0; t < kB; ++t) cur[t] = nxt[t];
+        }
+    }
+    return l2 ? acc : (metric == kIP ? -acc : 1.0 - acc);
+}
+
+}  // namespace hm
