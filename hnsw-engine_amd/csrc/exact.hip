@@ -517,6 +517,9 @@ struct TilePlan {
 
 // Tile-shape overrides of the range search (hm_set_exact_tuning; 0: automatic), process-wide.
 std::atomic<int> g_tile_qb{0}, g_tile_sample{0};
+// Float data on the float64 chunked kernels only (hm_set_exact_tuning with query_blocks -1), for
+// tests that compare the two paths.
+std::atomic<bool> g_float_chunked{false};
 
 TilePlan plan_tile(int64_t m, int64_t nqb, int nch, int k) {
     TilePlan p;
@@ -547,6 +550,221 @@ TilePlan plan_tile(int64_t m, int64_t nqb, int nch, int k) {
     return p;
 }
 
+// Padded width of the float range search for dim (multiples of 8 floats, of 32 above 128), or 0
+// when it does not apply.
+int ftile_width(int dim) {
+    int64_t kp = round_up(dim, 8);
+    if (kp > 128) kp = round_up(kp, 32);
+    return kp <= 256 ? (int)kp : 0;
+}
+
+TilePlan plan_ftile(int64_t m, int64_t nqb, int kp, int k) {
+    TilePlan p;
+    p.qb = ftile_qb(kp);
+    p.groups = (int)((nqb + kTileWaves * p.qb * 32 - 1) / (kTileWaves * p.qb * 32));
+    p.nchunks = (int)std::max<int64_t>(1, std::min<int64_t>(1024 / p.groups, m / 4096));
+    p.nchunks = std::min(p.nchunks, kTileMaxChunks);
+    p.chunk = round_up((m + p.nchunks - 1) / p.nchunks, kTileRows);
+    p.nchunks = (int)((m + p.chunk - 1) / p.chunk);
+    const int sbr = kp <= 64 ? 128 : (kp <= 128 ? 64 : 32);  // rows per superblock
+    const int nsb = (int)(p.chunk / sbr);
+    p.sample_every = std::max(1, std::min(4, nsb / 2));
+    if (g_tile_sample.load() > 0) p.sample_every = std::min(g_tile_sample.load(), nsb);
+    p.T = k <= 32 ? 4 : 8;
+    // As for the 8-bit plan; the margin admits a few more rows than the bare sample would.
+    const double mu = 2.0 * k * p.sample_every / (2.0 * p.nchunks);
+    p.E = (int)std::min(256.0, std::max(8.0, std::ceil(mu + 4.0 * std::sqrt(mu) + 4.0)));
+    p.cap = std::min(8192, std::max(1024, pow2_at_least(4 * k * p.sample_every)));
+    return p;
+}
+
+// Float data: the range search over float32 matrix-core tiles (exact_ftile.hip); the queries it
+// flags re-run on the float64 chunked kernels. Sets done = false (and returns HM_OK) when the
+// input is outside what the tiles cover (too few rows, too wide, or magnitudes near float32
+// overflow), for the caller to run the chunked kernels instead.
+int exact_float_tiles(const ExactArgs &a, int kp, const hipDeviceProp_t &prop, int64_t *d_ids,
+                      double *d_dists, hipStream_t s, bool &done) {
+    done = false;
+    const int64_t n = a.n, nq = a.nq;
+    const int dim = a.dim, k = a.k, metric = a.metric;
+    int64_t m = n;
+    Scoped<int32_t> idmap;
+    Scoped<int64_t> dcount;
+    Scoped<float> xc;
+    const float *xs = a.x32;
+    int64_t ld = dim;
+    int xdim = dim;
+    if (a.allowed) {
+        HM_TRY(idmap.alloc(n), "alloc");
+        HM_TRY(dcount.alloc(1), "alloc");
+        HM_TRY(subset_ids_launch(a.allowed, n, idmap.p, dcount.p, s), "subset launch");
+        HM_TRY(hipMemcpyAsync(&m, dcount.p, 8, hipMemcpyDeviceToHost, s), "copy count");
+        HM_TRY(hipStreamSynchronize(s), "subset count");
+        if (m < kTileMinRows || m < k) return HM_OK;
+    }
+    if (a.allowed || dim % 4 != 0 || (uintptr_t)a.x32 % 16 != 0) {
+        HM_TRY(xc.alloc((size_t)m * kp), "alloc");
+        HM_TRY(fgather_rows_launch(a.x32, dim, a.allowed ? idmap.p : nullptr, m, kp, xc.p, s),
+               "gather launch");
+        xs = xc.p;
+        ld = kp;
+        xdim = kp;
+    }
+    Scoped<float> cinit, rscale, qp;
+    Scoped<double> qnorm;
+    Scoped<unsigned long long> maxsq;
+    HM_TRY(cinit.alloc(m), "alloc");
+    if (metric == HM_COSINE) HM_TRY(rscale.alloc(m), "alloc");
+    HM_TRY(maxsq.alloc(1), "alloc");
+    HM_TRY(qp.alloc((size_t)nq * kp), "alloc");
+    HM_TRY(qnorm.alloc(nq), "alloc");
+    HM_TRY(hipMemsetAsync(maxsq.p, 0, 8, s), "clear");
+    HM_TRY(fprep_rows_launch(xs, ld, xdim, m, metric, cinit.p, rscale.p, maxsq.p, s), "prep launch");
+    HM_TRY(fprep_queries_launch(a.q32, nq, dim, kp, metric, qp.p, qnorm.p, s), "prep launch");
+    // Magnitudes: the float32 scores must stay far from overflow.
+    unsigned long long msq_bits = 0;
+    std::vector<double> hq((size_t)nq);
+    HM_TRY(hipMemcpyAsync(&msq_bits, maxsq.p, 8, hipMemcpyDeviceToHost, s), "copy norms");
+    HM_TRY(hipMemcpyAsync(hq.data(), qnorm.p, (size_t)nq * 8, hipMemcpyDeviceToHost, s), "copy norms");
+    HM_TRY(hipStreamSynchronize(s), "prep");
+    double msq;
+    memcpy(&msq, &msq_bits, 8);
+    const double qmax = *std::max_element(hq.begin(), hq.end());
+    if (!(msq < 1e30) || !(qmax * std::sqrt(msq) < 1e30)) return HM_OK;
+
+    Scoped<int32_t> bnd, trig, rec_mask, rec_row, rec_cnt, flags;
+    Scoped<float> V;
+    HM_TRY(flags.alloc(nq), "alloc");
+    size_t cap_bnd = 0, cap_rec = 0, cap_cnt = 0, cap_q = 0;
+    Events ev;
+    for (hipEvent_t &x : ev.e) HM_TRY(hipEventCreate(&x), "event");
+    double times[3] = {0, 0, 0};
+    for (int64_t b0 = 0; b0 < nq; b0 += kTileQueriesPerLaunch) {
+        const int64_t nqb = std::min<int64_t>(kTileQueriesPerLaunch, nq - b0);
+        const TilePlan p = plan_ftile(m, nqb, kp, k);
+        const int per_query = p.nchunks * 2 * p.T;
+        if (per_query < 2 * k) return HM_OK;
+        const size_t slots = (size_t)nqb * p.nchunks * 2;
+        if (slots * p.T > cap_bnd) {
+            HM_TRY(bnd.alloc(slots * p.T), "alloc");
+            cap_bnd = slots * p.T;
+        }
+        if (slots * p.E > cap_rec) {
+            HM_TRY(rec_mask.alloc(slots * p.E), "alloc");
+            HM_TRY(rec_row.alloc(slots * p.E), "alloc");
+            cap_rec = slots * p.E;
+        }
+        if (slots > cap_cnt) {
+            HM_TRY(rec_cnt.alloc(slots), "alloc");
+            cap_cnt = slots;
+        }
+        if ((size_t)nqb > cap_q) {
+            HM_TRY(trig.alloc(nqb), "alloc");
+            HM_TRY(V.alloc(nqb), "alloc");
+            cap_q = nqb;
+        }
+        FTileArgs t;
+        memset(&t, 0, sizeof(t));
+        t.x = xs;
+        t.ld = ld;
+        t.dim = xdim;
+        t.rscale = rscale.p;
+        t.cinit = cinit.p;
+        t.n = m;
+        t.q = qp.p + (size_t)b0 * kp;
+        t.nq = (int)nqb;
+        t.chunk = p.chunk;
+        t.nchunks = p.nchunks;
+        t.sample_every = p.sample_every;
+        t.bnd = bnd.p;
+        t.trig = V.p;
+        t.E = p.E;
+        t.rec_mask = rec_mask.p;
+        t.rec_row = rec_row.p;
+        t.rec_cnt = rec_cnt.p;
+        const dim3 grid(p.nchunks, p.groups);
+        HM_TRY(hipEventRecord(ev.e[0], s), "event");
+        HM_TRY(ftile_launch(0, kp, p.T, t, grid, s), "bound launch");
+        HM_TRY(threshold_launch(bnd.p, per_query, k, trig.p, (int)nqb, s), "threshold launch");
+        HM_TRY(fmargin_launch(trig.p, qnorm.p + b0, maxsq.p, metric, kp, dim, (int)nqb, V.p, s),
+               "margin launch");
+        HM_TRY(hipEventRecord(ev.e[1], s), "event");
+        HM_TRY(ftile_launch(1, kp, 1, t, grid, s), "range launch");
+        HM_TRY(hipEventRecord(ev.e[2], s), "event");
+        FCollectArgs c;
+        memset(&c, 0, sizeof(c));
+        c.rec_mask = rec_mask.p;
+        c.rec_row = rec_row.p;
+        c.rec_cnt = rec_cnt.p;
+        c.idmap = a.allowed ? idmap.p : nullptr;
+        c.x = a.x32;
+        c.q = a.q32 + (size_t)b0 * dim;
+        c.n = m;
+        c.nchunks = p.nchunks;
+        c.E = p.E;
+        c.k = k;
+        c.metric = metric;
+        c.dim = dim;
+        c.cap = p.cap;
+        c.ids = d_ids + b0 * k;
+        c.dists = d_dists + b0 * k;
+        c.flag = flags.p + b0;
+        HM_TRY(fcollect_launch(c, (int)nqb, s), "collect launch");
+        HM_TRY(hipEventRecord(ev.e[3], s), "event");
+        HM_TRY(hipEventSynchronize(ev.e[3]), "range search");
+        for (int i = 0; i < 3; ++i) {
+            float ms = 0.f;
+            HM_TRY(hipEventElapsedTime(&ms, ev.e[i], ev.e[i + 1]), "event time");
+            times[i] += ms;
+        }
+    }
+    g_timing.bound_ms = times[0];
+    g_timing.range_ms = times[1];
+    g_timing.collect_ms = times[2];
+    std::vector<int32_t> hflag((size_t)nq);
+    HM_TRY(hipMemcpyAsync(hflag.data(), flags.p, (size_t)nq * 4, hipMemcpyDeviceToHost, s), "copy flags");
+    HM_TRY(hipStreamSynchronize(s), "range search");
+    std::vector<int32_t> redo;
+    for (int64_t i = 0; i < nq; ++i)
+        if (hflag[(size_t)i]) redo.push_back((int32_t)i);
+    g_timing.redo = (int64_t)redo.size();
+    done = true;
+    if (redo.empty()) return HM_OK;
+
+    // Flagged queries: the float64 chunked kernels over all rows (with the bitset when masked).
+    const int64_t nr = (int64_t)redo.size();
+    Scoped<int32_t> ridx;
+    Scoped<float> rq;
+    Scoped<double> dxn, dqn, rd;
+    Scoped<int64_t> rids;
+    HM_TRY(ridx.alloc(nr), "alloc");
+    HM_TRY(rq.alloc((size_t)nr * dim), "alloc");
+    HM_TRY(rids.alloc((size_t)nr * k), "alloc");
+    HM_TRY(rd.alloc((size_t)nr * k), "alloc");
+    HM_TRY(hipMemcpyAsync(ridx.p, redo.data(), (size_t)nr * 4, hipMemcpyHostToDevice, s), "copy list");
+    HM_TRY(fgather_rows_launch(a.q32, dim, ridx.p, nr, dim, rq.p, s), "gather launch");
+    ExactArgs r = a;
+    r.q32 = rq.p;
+    r.nq = (int)nr;
+    if (metric == HM_COSINE) {
+        HM_TRY(dxn.alloc((size_t)n * dim), "alloc");
+        HM_TRY(dqn.alloc((size_t)nr * dim), "alloc");
+        hipLaunchKernelGGL(normalise_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           a.x32, n, dim, dxn.p);
+        hipLaunchKernelGGL(normalise_kernel, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s,
+                           rq.p, nr, dim, dqn.p);
+        r.xn = dxn.p;
+        r.qn = dqn.p;
+    }
+    const int st = run_chunked(r, false, prop, rids.p, rd.p, s);
+    if (st != HM_OK) return st;
+    hipLaunchKernelGGL(exact_scatter_kernel, dim3((unsigned)((nr * k + 255) / 256)), dim3(256), 0, s,
+                       rids.p, rd.p, ridx.p, nr, k, d_ids, d_dists);
+    HM_TRY(hipGetLastError(), "scatter launch");
+    HM_TRY(hipStreamSynchronize(s), "exact search");
+    return HM_OK;
+}
+
 // The exact search with every buffer on the device. Integer data (int8): the range search over
 // matrix-core tiles (exact_tile.hip) when it applies, the chunked kernels otherwise and for the
 // queries it flags; anything else: the float64 chunked kernels.
@@ -568,6 +786,12 @@ int exact_device(const float *d_data, int64_t n, int dim, int metric, const floa
         Scoped<double> dxn, dqn;
         a.x32 = d_data;
         a.q32 = d_q;
+        const int kp = ftile_width(dim);
+        if (kp > 0 && k <= 128 && n >= kTileMinRows && !g_float_chunked.load()) {
+            bool done = false;
+            const int st = exact_float_tiles(a, kp, prop, d_ids, d_dists, s, done);
+            if (st != HM_OK || done) return st;
+        }
         if (metric == HM_COSINE) {
             HM_TRY(dxn.alloc((size_t)n * dim), "alloc");
             HM_TRY(dqn.alloc((size_t)nq * dim), "alloc");
@@ -825,10 +1049,11 @@ extern "C" int hm_exact_dev(const float *d_data, int64_t n, int32_t dim, int32_t
 }
 
 extern "C" int hm_set_exact_tuning(int32_t query_blocks, int32_t sample_every) {
-    if (!(query_blocks == 0 || query_blocks == 1 || query_blocks == 2 || query_blocks == 4) ||
-        sample_every < 0 || sample_every > 64)
+    if (!(query_blocks >= -1 && query_blocks <= 4 && query_blocks != 3) || sample_every < 0 ||
+        sample_every > 64)
         return invalid("tuning values out of range");
-    g_tile_qb.store(query_blocks);
+    g_float_chunked.store(query_blocks == -1);
+    g_tile_qb.store(std::max(0, query_blocks));
     g_tile_sample.store(sample_every);
     return HM_OK;
 }

@@ -53,4 +53,60 @@ hipError_t gather_rows_launch(const int8_t *x8, const int32_t *xsq, const int32_
                               const int32_t *idmap, int64_t m, int dimp, int8_t *ox8,
                               int32_t *oxsq, int32_t *oxsum, hipStream_t s);
 
+
+// ---------------------------------------------------------------- float data (exact_ftile.hip)
+
+// Range search over float32 rows with float32 matrix cores (v_mfma_f32_32x32x2_f32). A row's
+// score is s = c_r + x.q (x, q scaled to unit norm for cosine): c_r = -|x|^2 / 2 for l2, 0
+// otherwise, so the score rises as the distance falls. Scores are approximate; the threshold
+// carries a margin that covers their rounding error (exact_fmargin_kernel).
+struct FTileArgs {
+    const float *x;        // rows, row stride ld floats (ld % 4 == 0), dim % 4 == 0 values each
+    int64_t ld;
+    int dim;
+    const float *rscale;   // per row factor applied while staging (cosine), or null
+    const float *cinit;    // (n) accumulator start per row
+    int64_t n;             // valid rows; rows beyond score -inf
+    const float *q;        // (nq, 8 * KH) prepared queries, zero padded
+    int nq;
+    int64_t chunk;
+    int nchunks;
+    int sample_every;
+    int32_t *bnd;          // bound pass out: (nq, nchunks, 2, T) order keys of block scores
+    const float *trig;     // range pass: (nq) threshold V
+    int E;
+    int32_t *rec_mask;     // (nq, nchunks, 2, E) which of the lane's 16 rows reach V (bit 4g+t:
+                           // row 8g + 4 (lane / 32) + t of the block)
+    int32_t *rec_row;      // (nq, nchunks, 2, E) first row of the 32-row block
+    int32_t *rec_cnt;      // (nq, nchunks, 2) records written (> E: overflow)
+};
+
+struct FCollectArgs {
+    const int32_t *rec_mask, *rec_row, *rec_cnt;
+    const int32_t *idmap;  // compact row -> id, or null
+    const float *x;        // original rows (n_all, dim), for the float64 distances
+    const float *q;        // original queries of this launch (nq, dim)
+    int64_t n;             // rows scanned
+    int nchunks, E, k, metric, dim, cap;
+    int64_t *ids;
+    double *dists;
+    int32_t *flag;
+};
+
+// Kernel geometry: queries per work-group and the LDS bytes of one launch (padded width kp).
+int ftile_qb(int kp);
+int ftile_lds_bytes(int kp);
+hipError_t ftile_launch(int mode, int kp, int T, const FTileArgs &a, dim3 grid, hipStream_t s);
+hipError_t fprep_rows_launch(const float *x, int64_t ld, int dim, int64_t n, int metric,
+                             float *cinit, float *rscale, unsigned long long *max_sq,
+                             hipStream_t s);
+hipError_t fprep_queries_launch(const float *q, int64_t nq, int dim, int kp, int metric,
+                                float *qp, double *qnorm, hipStream_t s);
+hipError_t fmargin_launch(const int32_t *trig_key, const double *qnorm,
+                          const unsigned long long *max_sq, int metric, int kp, int dim, int nq,
+                          float *V, hipStream_t s);
+hipError_t fcollect_launch(const FCollectArgs &a, int nq, hipStream_t s);
+hipError_t fgather_rows_launch(const float *x, int dim, const int32_t *idmap, int64_t m, int kp,
+                               float *out, hipStream_t s);
+
 }  // namespace hm

@@ -19,6 +19,9 @@ import time
 
 import numpy as np
 
+# MI355X dense float32 matrix-core peak (TFLOP/s).
+FP32_PEAK_TFLOPS = 157.3
+
 
 def _best(fn, repeats: int = 3):
     out = fn()
@@ -90,7 +93,8 @@ def quantized(log=print) -> dict:
     (float64 code-space queries, the reference's batch_search_codes), then the ef candidates
     re-ranked by exact float64 distance (rerank = ef); recall against the exact top-10 (float64
     brute force over the float32 vectors); ef from 32 to 512."""
-    from .exact import exact_distances
+    from . import _native
+    from .exact import exact_distances, last_range_search
     from .quantization import ScalarQuantizer
     from .recall import recall_at_k
     from .workloads import WORKLOADS, load_or_build
@@ -98,7 +102,24 @@ def quantized(log=print) -> dict:
     w = WORKLOADS["angular_400k"]
     eng, data, queries = load_or_build(w, log=log)
     k = 10
-    t_exact, (true_ids, true_d) = _best(lambda: eng.exact_search(queries, k), 1)
+    t_exact, (true_ids, true_d) = _best(lambda: eng.exact_search(queries, k), 2)
+    tiles = last_range_search()
+    # The same ground truth on the float64 chunked kernels, for comparison (identical results).
+    _native.check(_native.lib().hm_set_exact_tuning(-1, 0))
+    try:
+        t_f64, (f_ids, f_d) = _best(lambda: eng.exact_search(queries, k), 1)
+    finally:
+        _native.check(_native.lib().hm_set_exact_tuning(0, 0))
+    if not (np.array_equal(f_ids, true_ids) and np.array_equal(f_d, true_d)):
+        raise RuntimeError("float32 range search and float64 kernels disagree")
+    kernel_ms = tiles["bound_ms"] + tiles["range_ms"] + tiles["collect_ms"]
+    flops = 2.0 * queries.shape[0] * data.shape[0] * data.shape[1]
+    exact = {"seconds": t_exact, "float64_kernels_seconds": t_f64, **tiles,
+             "roofline": {"bound": "mfma", "unit": "TFLOP/s (fp32)", "peak": FP32_PEAK_TFLOPS,
+                          "achieved": flops / (kernel_ms / 1e3) / 1e12 if kernel_ms else None,
+                          "range_pass": flops / (tiles["range_ms"] / 1e3) / 1e12
+                          if tiles["range_ms"] else None}}
+    log(f"[quantized] exact {json.dumps(exact)}")
     quant = ScalarQuantizer.fit(data)
     codes = quant.encode(data)
     qcode = quant.transform_queries(queries, w.metric)
@@ -116,7 +137,8 @@ def quantized(log=print) -> dict:
         rows.append(row)
         log(f"[quantized] {json.dumps(row)}")
     return {"workload": f"{w.name}: {w.n} x {w.dim} cosine, {queries.shape[0]} queries, k={k}",
-            "quantizer": {"step": float(quant.step)}, "exact_seconds": t_exact, "rows": rows}
+            "quantizer": {"step": float(quant.step)}, "exact_seconds": t_exact, "exact": exact,
+            "rows": rows}
 
 
 def mixed_batch(w, nq: int = 100_000):

@@ -60,7 +60,7 @@ def exact_search(data, queries, k: int, metric: str = "l2", *, query_block: int 
 
 def last_range_search() -> dict:
     """Kernel times (ms) and re-run count of the last ``exact_search`` on this thread that took
-    the int8 range-search path (``hm_exact_timing``); zeros when it took another path."""
+    a range-search path, int8 or float32 (``hm_exact_timing``); zeros when it took another."""
     out = np.zeros(4, dtype=np.float64)
     _native.check(_native.lib().hm_exact_timing(_native.ptr(out)))
     return {"bound_ms": float(out[0]), "range_ms": float(out[1]), "collect_ms": float(out[2]),

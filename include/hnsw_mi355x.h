@@ -203,16 +203,18 @@ int hm_exact_dev(const float *d_data, int64_t n, int32_t dim, int32_t metric,
 int hm_index_exact(hm_index *index, const float *queries, int64_t nq, int32_t k,
                    const uint8_t *allowed, int64_t *ids, double *dists);
 
-/* Tile shape of the int8 range search (no counterpart in the reference; 0 restores the automatic
- * choice), process-wide: query_blocks (1, 2 or 4) blocks of 32 queries per wave, and
- * sample_every, the bound pass's sampling of one 128- or 256-row superblock in that many. Neither
- * changes results: a looser threshold only admits more rows to the exact ranking. */
+/* Tile shape of the range searches (no counterpart in the reference; 0 restores the automatic
+ * choice), process-wide: query_blocks (1, 2 or 4) blocks of 32 queries per wave of the int8
+ * search, and sample_every, the bound pass's sampling of one superblock in that many (both the
+ * int8 and the float32 search). Neither changes results: a looser threshold only admits more rows
+ * to the exact ranking. query_blocks = -1 sends float data to the float64 chunked kernels instead
+ * of the float32 range search (same results; for comparing the two). */
 int hm_set_exact_tuning(int32_t query_blocks, int32_t sample_every);
 
 /* Instrumentation (no counterpart in the reference): kernel times in ms of the last exact
- * search on the calling thread that took the int8 range-search path, out[0] bound pass plus
- * threshold, out[1] range pass, out[2] collect, and out[3] the number of queries it re-ran on
- * the chunked kernels (all 0 when the last search took another path). */
+ * search on the calling thread that took a range-search path (int8 or float32), out[0] bound
+ * pass plus threshold, out[1] range pass, out[2] collect, and out[3] the number of queries it
+ * re-ran on the chunked kernels (all 0 when the last search took another path). */
 int hm_exact_timing(double *out);
 
 /* Sequential insertion (host C++, the reference's _insert_batch): inserts vectors[start..n)
