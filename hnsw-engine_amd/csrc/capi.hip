@@ -10,6 +10,7 @@
 
 #include "../../include/hnsw_mi355x.h"
 #include "common.hpp"
+#include "exact_tile.hpp"
 #include "index.hpp"
 #include "search.hpp"
 
@@ -661,10 +662,25 @@ int hm_open(const hm_native_graph *g, int device, hm_index **out) {
         for (int64_t i = 0; i < n; ++i)
             for (int j = 0; j < g->dim; ++j)
                 u[(size_t)i * h->dimp + j] = (uint8_t)g->vectors[(size_t)i * g->dim + j];
-        if ((e = h->u8.ensure(u.size())) != hipSuccess) return fail(fail_hip(e, "alloc"));
-        e = hipMemcpy(h->u8.p, u.data(), u.size(), hipMemcpyHostToDevice);
+        // Padding rows after the data let the exact search's tiles read the rows in place.
+        const int64_t rows_padded = n + tile_pad_rows(n);
+        const size_t padded = (size_t)rows_padded * h->dimp;
+        if ((e = h->u8.ensure(padded, true, h->stream)) != hipSuccess)
+            return fail(fail_hip(e, "alloc"));
+        e = hipMemcpyAsync(h->u8.p, u.data(), u.size(), hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) return fail(fail_hip(e, "upload u8"));
         h->exact_u8 = true;
+        // Row terms and accumulator starts of the exact search, computed once.
+        if ((e = h->ex_xsq.ensure(n)) != hipSuccess || (e = h->ex_xsum.ensure(n)) != hipSuccess ||
+            (e = h->ex_cinit.ensure(rows_padded)) != hipSuccess)
+            return fail(fail_hip(e, "alloc"));
+        const I8Rows rows{h->u8.p, h->dimp, 0x80808080u};
+        if ((e = row_terms_launch(rows, n, h->dim, h->ex_xsq.p, h->ex_xsum.p, h->stream)) != hipSuccess ||
+            (e = cinit_launch(h->ex_xsq.p, h->ex_xsum.p, n, rows_padded, h->metric,
+                              h->ex_cinit.p, h->stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(h->stream)) != hipSuccess)
+            return fail(fail_hip(e, "exact row terms"));
     }
     *out = h;
     return HM_OK;

@@ -24,6 +24,9 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/hnsw_mi355x.h"
@@ -44,8 +47,10 @@ struct ExactArgs {
     // float64 producer
     const float *x32, *q32;
     const double *xn, *qn;  // cosine: rows / queries divided by their norms
-    // int8 producer: (rows, dimp) shifted bytes, zero padded; per-row/query sums
-    const int8_t *x8, *q8;
+    // int8 producer: rows read as bytes ^ flip (shifted values), queries (nq, dimp) shifted and
+    // zero padded; per-row/query sums of the shifted values
+    I8Rows rows;
+    const int8_t *q8;
     const int32_t *xsq, *qsq, *xsum, *qsum;
     const uint32_t *allowed;  // bitset or null
     int64_t chunk;
@@ -189,7 +194,12 @@ __global__ __launch_bounds__(128) void exact_i8_kernel(ExactArgs a) {
 #pragma unroll
         for (int c = 0; c < MAXCH; ++c) {
             bv[c] = i32x4{0, 0, 0, 0};
-            if (c < nch && jv) bv[c] = *(const i32x4 *)(a.x8 + (size_t)j * a.dimp + c * 32 + half * 16);
+            const int off = c * 32 + half * 16;
+            if (c < nch && jv && off < a.rows.ld) {
+                const uint4 w = *(const uint4 *)(a.rows.x + (size_t)j * a.rows.ld + off);
+                const uint32_t f = a.rows.flip;
+                bv[c] = i32x4{(int)(w.x ^ f), (int)(w.y ^ f), (int)(w.z ^ f), (int)(w.w ^ f)};
+            }
         }
         i32x16 acc = {0};
 #pragma unroll
@@ -408,10 +418,6 @@ struct Scoped {
     }
 };
 
-// Rows of zero padding kept after the data so that any tile plan reads in bounds (a plan pads
-// each of at most kTileMaxChunks chunks by fewer than kTileRows rows).
-constexpr int kTileMaxChunks = 1024;
-constexpr int64_t kTilePadRows = (int64_t)kTileMaxChunks * kTileRows;
 constexpr int64_t kTileMinRows = 8192;     // smaller inputs go to the chunked kernels
 constexpr int kTileQueriesPerLaunch = 16384;
 
@@ -765,15 +771,41 @@ int exact_float_tiles(const ExactArgs &a, int kp, const hipDeviceProp_t &prop, i
     return HM_OK;
 }
 
-// The exact search with every buffer on the device. Integer data (int8): the range search over
-// matrix-core tiles (exact_tile.hip) when it applies, the chunked kernels otherwise and for the
-// queries it flags; anything else: the float64 chunked kernels.
-int exact_device(const float *d_data, int64_t n, int dim, int metric, const float *d_q,
-                 int64_t nq, int k, const uint32_t *d_bits, bool int8, int64_t *d_ids,
-                 double *d_dists, hipStream_t s, int device) {
-    g_timing = ExactTiming();
-    hipDeviceProp_t prop;
-    HM_TRY(hipGetDeviceProperties(&prop, device), "device properties");
+// Per-device buffers of the 8-bit exact search, kept between calls. A search holds the lock for
+// its whole run and waits for its stream before returning, so the next holder finds them idle.
+struct ExactScratch {
+    std::mutex mu;
+    DevBuf<int8_t> x8, q8, cx8;
+    DevBuf<int32_t> xsq, xsum, qsq, qsum, cinit, idmap, cxsq, cxsum;
+    DevBuf<int32_t> bnd, trig, rec_mask, rec_row, rec_cnt, flags;
+    DevBuf<int64_t> count;
+};
+
+ExactScratch &exact_scratch(int device) {
+    static std::mutex m;
+    static std::map<int, std::unique_ptr<ExactScratch>> all;
+    std::lock_guard<std::mutex> g(m);
+    std::unique_ptr<ExactScratch> &p = all[device];
+    if (!p) p.reset(new ExactScratch());
+    return *p;
+}
+
+// 8-bit rows and their per-row terms. cinit (the range search's accumulator starts over
+// n + tile_pad_rows(n) rows) is set when the rows are ready for the tiles as they are: ld equal
+// to the tiles' row width and readable up to n + tile_pad_rows(n).
+struct I8Data {
+    I8Rows rows;
+    const int32_t *xsq, *xsum, *cinit;
+};
+
+// Exact search over 8-bit rows with float32 queries holding 0..255 integers: the range search
+// over int8 matrix-core tiles (exact_tile.hip) when it applies, the chunked kernels otherwise and
+// for the queries it flags. The caller holds S's lock.
+int exact_int8(const I8Data &d, int64_t n, int dim, int metric, const float *d_q, int64_t nq,
+               int k, const uint32_t *d_bits, const hipDeviceProp_t &prop, ExactScratch &S,
+               int64_t *d_ids, double *d_dists, hipStream_t s) {
+    const bool tile_ok = dim <= 256 && k <= 128 && n >= kTileMinRows;
+    const int nch = tile_ok ? pow2_at_least((dim + 31) / 32) : 0;
     ExactArgs a;
     memset(&a, 0, sizeof(a));
     a.n = n;
@@ -782,89 +814,49 @@ int exact_device(const float *d_data, int64_t n, int dim, int metric, const floa
     a.k = k;
     a.nq = (int)nq;
     a.allowed = d_bits;
-    if (!int8) {
-        Scoped<double> dxn, dqn;
-        a.x32 = d_data;
-        a.q32 = d_q;
-        const int kp = ftile_width(dim);
-        if (kp > 0 && k <= 128 && n >= kTileMinRows && !g_float_chunked.load()) {
-            bool done = false;
-            const int st = exact_float_tiles(a, kp, prop, d_ids, d_dists, s, done);
-            if (st != HM_OK || done) return st;
-        }
-        if (metric == HM_COSINE) {
-            HM_TRY(dxn.alloc((size_t)n * dim), "alloc");
-            HM_TRY(dqn.alloc((size_t)nq * dim), "alloc");
-            hipLaunchKernelGGL(normalise_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                               d_data, n, dim, dxn.p);
-            hipLaunchKernelGGL(normalise_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s,
-                               d_q, nq, dim, dqn.p);
-            a.xn = dxn.p;
-            a.qn = dqn.p;
-        }
-        return run_chunked(a, false, prop, d_ids, d_dists, s);
-    }
-
-    const bool tile_ok = dim <= 256 && k <= 128 && n >= kTileMinRows;
-    const int nch = tile_ok ? pow2_at_least((dim + 31) / 32) : 0;
     a.dimp = tile_ok ? 32 * nch : (int)round_up(dim, 32);
-    const int64_t rows_alloc = tile_ok ? n + kTilePadRows : n;
-    Scoped<int8_t> x8, q8;
-    Scoped<int32_t> xsq, xsum, qsq, qsum;
-    HM_TRY(x8.alloc((size_t)rows_alloc * a.dimp), "alloc");
-    HM_TRY(q8.alloc((size_t)nq * a.dimp), "alloc");
-    HM_TRY(xsq.alloc(rows_alloc), "alloc");
-    HM_TRY(xsum.alloc(rows_alloc), "alloc");
-    HM_TRY(qsq.alloc(nq), "alloc");
-    HM_TRY(qsum.alloc(nq), "alloc");
-    hipLaunchKernelGGL(shift_i8_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, d_data, n,
-                       dim, a.dimp, x8.p, xsq.p, xsum.p);
+    a.rows = d.rows;
+    a.xsq = d.xsq;
+    a.xsum = d.xsum;
+    HM_TRY(S.q8.ensure((size_t)nq * a.dimp), "alloc");
+    HM_TRY(S.qsq.ensure(nq), "alloc");
+    HM_TRY(S.qsum.ensure(nq), "alloc");
     hipLaunchKernelGGL(shift_i8_kernel, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, d_q, nq,
-                       dim, a.dimp, q8.p, qsq.p, qsum.p);
+                       dim, a.dimp, S.q8.p, S.qsq.p, S.qsum.p);
     HM_TRY(hipGetLastError(), "shift launch");
-    a.x8 = x8.p;
-    a.q8 = q8.p;
-    a.xsq = xsq.p;
-    a.xsum = xsum.p;
-    a.qsq = qsq.p;
-    a.qsum = qsum.p;
+    a.q8 = S.q8.p;
+    a.qsq = S.qsq.p;
+    a.qsum = S.qsum.p;
     if (!tile_ok) return run_chunked(a, true, prop, d_ids, d_dists, s);
-    if (rows_alloc > n)
-        HM_TRY(hipMemsetAsync(x8.p + (size_t)n * a.dimp, 0, (size_t)(rows_alloc - n) * a.dimp, s),
-               "clear padding");
 
-    // Masked search: the allowed rows, compacted in id order.
+    // The rows the tiles scan: as given when ready for them; otherwise (masked: the allowed
+    // rows, compacted in id order) gathered into shifted rows of the tile width, padded.
     int64_t m = n;
-    const int8_t *tx8 = x8.p;
-    const int32_t *txsq = xsq.p, *txsum = xsum.p;
-    Scoped<int32_t> idmap, cxsq, cxsum;
-    Scoped<int8_t> cx8;
-    Scoped<int64_t> dcount;
+    I8Rows trows = d.rows;
+    const int32_t *tcinit = d.cinit;
     if (d_bits) {
-        HM_TRY(idmap.alloc(n), "alloc");
-        HM_TRY(dcount.alloc(1), "alloc");
-        HM_TRY(subset_ids_launch(d_bits, n, idmap.p, dcount.p, s), "subset launch");
-        HM_TRY(hipMemcpyAsync(&m, dcount.p, 8, hipMemcpyDeviceToHost, s), "copy count");
+        HM_TRY(S.idmap.ensure(n), "alloc");
+        HM_TRY(S.count.ensure(1), "alloc");
+        HM_TRY(subset_ids_launch(d_bits, n, S.idmap.p, S.count.p, s), "subset launch");
+        HM_TRY(hipMemcpyAsync(&m, S.count.p, 8, hipMemcpyDeviceToHost, s), "copy count");
         HM_TRY(hipStreamSynchronize(s), "subset count");
         if (m < kTileMinRows || m < k) return run_chunked(a, true, prop, d_ids, d_dists, s);
-        HM_TRY(cx8.alloc((size_t)(m + kTilePadRows) * a.dimp), "alloc");
-        HM_TRY(cxsq.alloc(m + kTilePadRows), "alloc");
-        HM_TRY(cxsum.alloc(m + kTilePadRows), "alloc");
-        HM_TRY(hipMemsetAsync(cx8.p + (size_t)m * a.dimp, 0, (size_t)kTilePadRows * a.dimp, s),
-               "clear padding");
-        HM_TRY(gather_rows_launch(x8.p, xsq.p, xsum.p, idmap.p, m, a.dimp, cx8.p, cxsq.p, cxsum.p, s),
-               "gather launch");
-        tx8 = cx8.p;
-        txsq = cxsq.p;
-        txsum = cxsum.p;
     }
-    Scoped<int32_t> cinit, flags;
-    HM_TRY(cinit.alloc(m + kTilePadRows), "alloc");
-    HM_TRY(flags.alloc(nq), "alloc");
-    HM_TRY(cinit_launch(txsq, txsum, m, m + kTilePadRows, metric, cinit.p, s), "cinit launch");
+    if (d_bits || !tcinit || d.rows.ld != a.dimp) {
+        HM_TRY(S.cx8.ensure((size_t)(m + tile_pad_rows(m)) * a.dimp), "alloc");
+        HM_TRY(S.cxsq.ensure(m), "alloc");
+        HM_TRY(S.cxsum.ensure(m), "alloc");
+        HM_TRY(gather_rows_launch(d.rows, d.xsq, d.xsum, d_bits ? S.idmap.p : nullptr, m, a.dimp,
+                                  S.cx8.p, S.cxsq.p, S.cxsum.p, s),
+               "gather launch");
+        trows = I8Rows{(const uint8_t *)S.cx8.p, a.dimp, 0u};
+        HM_TRY(S.cinit.ensure(m + tile_pad_rows(m)), "alloc");
+        HM_TRY(cinit_launch(S.cxsq.p, S.cxsum.p, m, m + tile_pad_rows(m), metric, S.cinit.p, s),
+               "cinit launch");
+        tcinit = S.cinit.p;
+    }
+    HM_TRY(S.flags.ensure(nq), "alloc");
 
-    Scoped<int32_t> bnd, trig, rec_val, rec_row, rec_cnt;
-    size_t cap_bnd = 0, cap_rec = 0, cap_cnt = 0, cap_q = 0;
     Events ev;
     for (hipEvent_t &x : ev.e) HM_TRY(hipEventCreate(&x), "event");
     double times[3] = {0, 0, 0};
@@ -874,38 +866,26 @@ int exact_device(const float *d_data, int64_t n, int dim, int metric, const floa
         const int per_query = p.nchunks * 2 * p.T;
         if (per_query < 2 * k) return run_chunked(a, true, prop, d_ids, d_dists, s);
         const size_t slots = (size_t)nqb * p.nchunks * 2;
-        if (slots * p.T > cap_bnd) {
-            HM_TRY(bnd.alloc(slots * p.T), "alloc");
-            cap_bnd = slots * p.T;
-        }
-        if (slots * p.E > cap_rec) {
-            HM_TRY(rec_val.alloc(slots * p.E * 16), "alloc");
-            HM_TRY(rec_row.alloc(slots * p.E), "alloc");
-            cap_rec = slots * p.E;
-        }
-        if (slots > cap_cnt) {
-            HM_TRY(rec_cnt.alloc(slots), "alloc");
-            cap_cnt = slots;
-        }
-        if ((size_t)nqb > cap_q) {
-            HM_TRY(trig.alloc(nqb), "alloc");
-            cap_q = nqb;
-        }
+        HM_TRY(S.bnd.ensure(slots * p.T), "alloc");
+        HM_TRY(S.rec_mask.ensure(slots * p.E), "alloc");
+        HM_TRY(S.rec_row.ensure(slots * p.E), "alloc");
+        HM_TRY(S.rec_cnt.ensure(slots), "alloc");
+        HM_TRY(S.trig.ensure(nqb), "alloc");
         TileArgs t;
         memset(&t, 0, sizeof(t));
-        t.x8 = tx8;
-        t.cinit = cinit.p;
-        t.q8 = q8.p + (size_t)b0 * a.dimp;
+        t.rows = trows;
+        t.cinit = tcinit;
+        t.q8 = S.q8.p + (size_t)b0 * a.dimp;
         t.nq = (int)nqb;
         t.chunk = p.chunk;
         t.nchunks = p.nchunks;
         t.sample_every = p.sample_every;
-        t.bnd = bnd.p;
-        t.trig = trig.p;
+        t.bnd = S.bnd.p;
+        t.trig = S.trig.p;
         t.E = p.E;
-        t.rec_val = rec_val.p;
-        t.rec_row = rec_row.p;
-        t.rec_cnt = rec_cnt.p;
+        t.rec_mask = S.rec_mask.p;
+        t.rec_row = S.rec_row.p;
+        t.rec_cnt = S.rec_cnt.p;
         const dim3 grid(p.nchunks, p.groups);
         // The bound pass with eight list entries per query block keeps fewer query blocks per
         // wave at 128 dimensions (register budget); its output does not depend on the grouping.
@@ -914,19 +894,19 @@ int exact_device(const float *d_data, int64_t n, int dim, int metric, const floa
                                                     (kTileWaves * qb_bound * 32)));
         HM_TRY(hipEventRecord(ev.e[0], s), "event");
         HM_TRY(tile_launch(0, p.nch, qb_bound, p.T, t, grid_bound, s), "bound launch");
-        HM_TRY(threshold_launch(bnd.p, per_query, k, trig.p, (int)nqb, s), "threshold launch");
+        HM_TRY(threshold_launch(S.bnd.p, per_query, k, S.trig.p, (int)nqb, s), "threshold launch");
         HM_TRY(hipEventRecord(ev.e[1], s), "event");
         HM_TRY(tile_launch(1, p.nch, p.qb, p.T, t, grid, s), "range launch");
         HM_TRY(hipEventRecord(ev.e[2], s), "event");
         CollectArgs c;
         memset(&c, 0, sizeof(c));
-        c.rec_val = rec_val.p;
-        c.rec_row = rec_row.p;
-        c.rec_cnt = rec_cnt.p;
-        c.trig = trig.p;
-        c.xsq = txsq;
-        c.idmap = d_bits ? idmap.p : nullptr;
-        c.qterm = (metric == HM_L2 ? qsq.p : qsum.p) + b0;
+        c.rec_mask = S.rec_mask.p;
+        c.rec_row = S.rec_row.p;
+        c.rec_cnt = S.rec_cnt.p;
+        c.rows = trows;
+        c.q8 = S.q8.p + (size_t)b0 * a.dimp;
+        c.qld = a.dimp;
+        c.idmap = d_bits ? S.idmap.p : nullptr;
         c.n = m;
         c.nchunks = p.nchunks;
         c.E = p.E;
@@ -936,7 +916,7 @@ int exact_device(const float *d_data, int64_t n, int dim, int metric, const floa
         c.cap = p.cap;
         c.ids = d_ids + b0 * k;
         c.dists = d_dists + b0 * k;
-        c.flag = flags.p + b0;
+        c.flag = S.flags.p + b0;
         HM_TRY(collect_launch(c, (int)nqb, s), "collect launch");
         HM_TRY(hipEventRecord(ev.e[3], s), "event");
         HM_TRY(hipEventSynchronize(ev.e[3]), "range search");
@@ -950,7 +930,8 @@ int exact_device(const float *d_data, int64_t n, int dim, int metric, const floa
     g_timing.range_ms = times[1];
     g_timing.collect_ms = times[2];
     std::vector<int32_t> hflag((size_t)nq);
-    HM_TRY(hipMemcpyAsync(hflag.data(), flags.p, (size_t)nq * 4, hipMemcpyDeviceToHost, s), "copy flags");
+    HM_TRY(hipMemcpyAsync(hflag.data(), S.flags.p, (size_t)nq * 4, hipMemcpyDeviceToHost, s),
+           "copy flags");
     HM_TRY(hipStreamSynchronize(s), "range search");
     std::vector<int32_t> redo;
     for (int64_t i = 0; i < nq; ++i)
@@ -971,7 +952,8 @@ int exact_device(const float *d_data, int64_t n, int dim, int metric, const floa
     HM_TRY(rids.alloc((size_t)nr * k), "alloc");
     HM_TRY(rd.alloc((size_t)nr * k), "alloc");
     HM_TRY(hipMemcpyAsync(ridx.p, redo.data(), (size_t)nr * 4, hipMemcpyHostToDevice, s), "copy list");
-    HM_TRY(gather_rows_launch(q8.p, qsq.p, qsum.p, ridx.p, nr, a.dimp, rq8.p, rqsq.p, rqsum.p, s),
+    HM_TRY(gather_rows_launch(I8Rows{(const uint8_t *)S.q8.p, a.dimp, 0u}, S.qsq.p, S.qsum.p,
+                              ridx.p, nr, a.dimp, rq8.p, rqsq.p, rqsum.p, s),
            "gather launch");
     ExactArgs r = a;
     r.q8 = rq8.p;
@@ -985,6 +967,88 @@ int exact_device(const float *d_data, int64_t n, int dim, int metric, const floa
     HM_TRY(hipGetLastError(), "scatter launch");
     HM_TRY(hipStreamSynchronize(s), "exact search");
     return HM_OK;
+}
+
+// The exact search with every buffer on the device. Integer data (int8): rows shifted to int8
+// into per-device scratch, then exact_int8; anything else: the float32 range search when it
+// applies, the float64 chunked kernels otherwise.
+int exact_device(const float *d_data, int64_t n, int dim, int metric, const float *d_q,
+                 int64_t nq, int k, const uint32_t *d_bits, bool int8, int64_t *d_ids,
+                 double *d_dists, hipStream_t s, int device) {
+    g_timing = ExactTiming();
+    hipDeviceProp_t prop;
+    HM_TRY(hipGetDeviceProperties(&prop, device), "device properties");
+    if (!int8) {
+        ExactArgs a;
+        memset(&a, 0, sizeof(a));
+        a.n = n;
+        a.dim = dim;
+        a.metric = metric;
+        a.k = k;
+        a.nq = (int)nq;
+        a.allowed = d_bits;
+        a.x32 = d_data;
+        a.q32 = d_q;
+        const int kp = ftile_width(dim);
+        if (kp > 0 && k <= 128 && n >= kTileMinRows && !g_float_chunked.load()) {
+            bool done = false;
+            const int st = exact_float_tiles(a, kp, prop, d_ids, d_dists, s, done);
+            if (st != HM_OK || done) return st;
+        }
+        Scoped<double> dxn, dqn;
+        if (metric == HM_COSINE) {
+            HM_TRY(dxn.alloc((size_t)n * dim), "alloc");
+            HM_TRY(dqn.alloc((size_t)nq * dim), "alloc");
+            hipLaunchKernelGGL(normalise_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                               d_data, n, dim, dxn.p);
+            hipLaunchKernelGGL(normalise_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s,
+                               d_q, nq, dim, dqn.p);
+            a.xn = dxn.p;
+            a.qn = dqn.p;
+        }
+        return run_chunked(a, false, prop, d_ids, d_dists, s);
+    }
+    ExactScratch &S = exact_scratch(device);
+    std::lock_guard<std::mutex> lock(S.mu);
+    // Rows shifted to int8 at the width the tiles read, padded (a mask or the chunked kernels
+    // take them as they are).
+    const bool tile_ok = dim <= 256 && k <= 128 && n >= kTileMinRows;
+    const int ld = tile_ok ? 32 * pow2_at_least((dim + 31) / 32) : (int)round_up(dim, 32);
+    const int64_t rows = tile_ok ? n + tile_pad_rows(n) : n;
+    HM_TRY(S.x8.ensure((size_t)rows * ld), "alloc");
+    HM_TRY(S.xsq.ensure(n), "alloc");
+    HM_TRY(S.xsum.ensure(n), "alloc");
+    hipLaunchKernelGGL(shift_i8_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, d_data, n,
+                       dim, ld, S.x8.p, S.xsq.p, S.xsum.p);
+    HM_TRY(hipGetLastError(), "shift launch");
+    const int32_t *cinit = nullptr;
+    if (tile_ok && !d_bits) {
+        HM_TRY(S.cinit.ensure(rows), "alloc");
+        HM_TRY(cinit_launch(S.xsq.p, S.xsum.p, n, rows, metric, S.cinit.p, s), "cinit launch");
+        cinit = S.cinit.p;
+    }
+    const I8Data d{I8Rows{(const uint8_t *)S.x8.p, ld, 0u}, S.xsq.p, S.xsum.p, cinit};
+    return exact_int8(d, n, dim, metric, d_q, nq, k, d_bits, prop, S, d_ids, d_dists, s);
+}
+
+// The exact search on an index handle's rows (queries, mask and outputs on the device): the
+// handle's raw 8-bit rows and cached row terms when it holds them and the queries are 0..255
+// integers, its float32 rows otherwise.
+int exact_handle(hm_index *h, const float *d_q, int64_t nq, int k, const uint32_t *d_bits,
+                 bool int8, int64_t *d_ids, double *d_dists) {
+    if (!int8) {
+        return exact_device(h->vec.p, h->n, h->dim, h->metric, d_q, nq, k, d_bits, false, d_ids,
+                            d_dists, h->stream, h->device);
+    }
+    g_timing = ExactTiming();
+    hipDeviceProp_t prop;
+    HM_TRY(hipGetDeviceProperties(&prop, h->device), "device properties");
+    ExactScratch &S = exact_scratch(h->device);
+    std::lock_guard<std::mutex> lock(S.mu);
+    const I8Data d{I8Rows{h->u8.p, h->dimp, 0x80808080u}, h->ex_xsq.p, h->ex_xsum.p,
+                   h->ex_cinit.p};
+    return exact_int8(d, h->n, h->dim, h->metric, d_q, nq, k, d_bits, prop, S, d_ids, d_dists,
+                      h->stream);
 }
 
 int check_exact(int64_t n, int32_t dim, int32_t metric, int32_t k) {
@@ -1075,8 +1139,7 @@ extern "C" int hm_index_exact(hm_index *h, const float *queries, int64_t nq, int
     if (st != HM_OK) return st;
     if (nq == 0) return HM_OK;
     HM_TRY(hipSetDevice(h->device), "hipSetDevice");
-    const bool int8 = h->exact_u8 && h->metric != HM_COSINE && h->dim <= 1024 &&
-                      u8_values(queries, (size_t)nq * h->dim);
+    const bool int8 = h->exact_u8 && u8_values(queries, (size_t)nq * h->dim);
     Scoped<float> dq;
     Scoped<uint32_t> bits;
     Scoped<int64_t> oid;
@@ -1094,12 +1157,24 @@ extern "C" int hm_index_exact(hm_index *h, const float *queries, int64_t nq, int
         HM_TRY(hipMemcpyAsync(bits.p, b.data(), b.size() * 4, hipMemcpyHostToDevice, h->stream),
                "copy mask");
     }
-    const int rc = exact_device(h->vec.p, h->n, h->dim, h->metric, dq.p, nq, k,
-                                allowed ? bits.p : nullptr, int8, oid.p, od.p, h->stream, h->device);
+    const int rc = exact_handle(h, dq.p, nq, k, allowed ? bits.p : nullptr, int8, oid.p, od.p);
     if (rc != HM_OK) return rc;
     HM_TRY(hipMemcpyAsync(ids, oid.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, h->stream), "copy ids");
     HM_TRY(hipMemcpyAsync(dists, od.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, h->stream),
            "copy dists");
     HM_TRY(hipStreamSynchronize(h->stream), "exact search");
     return HM_OK;
+}
+
+extern "C" int hm_index_exact_dev(hm_index *h, const float *d_queries, int64_t nq, int32_t k,
+                                  const uint32_t *d_allowed_bits, int queries_u8, int64_t *d_ids,
+                                  double *d_dists) {
+    if (!h || !d_queries || !d_ids || !d_dists) return invalid("null argument");
+    if (h->n == 0) return invalid("cannot search an empty index");
+    const int st = check_exact(h->n, h->dim, h->metric, k);
+    if (st != HM_OK) return st;
+    if (nq == 0) return HM_OK;
+    HM_TRY(hipSetDevice(h->device), "hipSetDevice");
+    return exact_handle(h, d_queries, nq, k, d_allowed_bits, h->exact_u8 && queries_u8 != 0,
+                        d_ids, d_dists);
 }

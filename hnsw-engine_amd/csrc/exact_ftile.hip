@@ -143,7 +143,10 @@ __global__ __launch_bounds__(kTileWaves * 64) void exact_ftile_kernel(FTileArgs 
             fetch(base_, tid + 2 * THREADS, pre2, rs2);                                \
         if (LPT > 3 && tid + 3 * THREADS < ELEMS)                                      \
             fetch(base_, tid + 3 * THREADS, pre3, rs3);                                \
-        if (tid < SBR) pci = base_ + tid < a.n ? a.cinit[base_ + tid] : -INFINITY;     \
+        if (tid < SBR) {                                                               \
+            pci = -INFINITY;                                                           \
+            if (base_ + tid < a.n) pci = a.cinit[base_ + tid];                         \
+        }                                                                              \
     } while (0)
     auto stash_one = [&](int buf, int e, float4 v, float rs) {
         const int r = e / SLOTS, sl = e % SLOTS;
@@ -379,7 +382,6 @@ __global__ __launch_bounds__(64) void exact_fcollect_kernel(FCollectArgs a) {
                 row0[j] = a.rec_row[rec];
                 mask[j] = a.rec_mask[rec];
             }
-#pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int half = (base + j * 64 + lane) & 1;
                 uint32_t m = (uint32_t)mask[j];

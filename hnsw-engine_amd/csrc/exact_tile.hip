@@ -12,9 +12,11 @@
 //             holds for one 32-row MFMA block);
 //   threshold per query, V = the k-th largest of those block scores. They belong to disjoint
 //             row groups, so at least k rows score >= V, and every true k-nearest row does;
-//   range     all rows: a lane whose block score reaches V writes its 16 scores to a
-//             per-(query, chunk, half) record list (E records; more sets an overflow count);
-//   collect   per query: the record rows scoring >= V, exact distances, sort by (d, id), top k.
+//   range     all rows: a lane whose block score reaches V writes the block's first row and a
+//             mask of its 16 rows that reach V to a per-(query, chunk, half) record list
+//             (E records; more sets an overflow count);
+//   collect   per query: the masked rows, exact distances recomputed from the rows, sort by
+//             (d, id), top k.
 //             A query whose records overflowed or whose candidates exceed the collect buffer is
 //             flagged and re-run by the general kernel.
 //
@@ -24,6 +26,9 @@
 //   ip: c_r = 128 sum(x'); x.q = score + 128 sum(q') + 128^2 dim, d = -x.q.
 // For l2 a row with d <= d(V) := |q'|^2 - 2V has 2 score + 1 >= 2V, so score >= V: the range
 // pass misses no row the threshold admits.
+//
+// Rows are read as bytes ^ flip (I8Rows), so the search runs on raw 0..255 rows held by an
+// index handle (flip 0x80 per byte) as well as on rows shifted per call (flip 0).
 //
 // Tiling: a work-group is 8 waves (two per SIMD, so one wave's matrix-core work overlaps the
 // other's compares) over QB x 32 queries each (B operands in registers for the whole launch) and
@@ -108,11 +113,15 @@ __global__ __launch_bounds__(kTileWaves * 64) void exact_tile_kernel(TileArgs a)
     static_assert(LPT <= 4, "at most four 16-byte loads per thread and superblock");
     uint4 pre0 = uint4{0, 0, 0, 0}, pre1 = pre0, pre2 = pre0, pre3 = pre0;
     int4 pci = int4{0, 0, 0, 0};
+    // Rows are ROWB bytes apart and readable up to the end of every chunk (rows at and beyond n
+    // are padding whose accumulator start keeps them below any threshold). The shift (^ flip)
+    // waits for the stash, so the loads stay in flight through this superblock's matrix work.
+    const uint32_t flip = a.rows.flip;
 #define HM_TILE_LOAD(S)                                                                       \
     do {                                                                                      \
         const int sb_ = MODE == 0 ? (S) * a.sample_every : (S);                               \
         const int64_t base_ = r0 + (int64_t)sb_ * SBR;                                        \
-        const int8_t *src_ = a.x8 + (size_t)base_ * ROWB;                                     \
+        const uint8_t *src_ = a.rows.x + (size_t)base_ * ROWB;                                \
         if (tid < ELEMS)                                                                      \
             pre0 = *(const uint4 *)(src_ + (size_t)(tid / SLOTS) * ROWB + (tid % SLOTS) * 16); \
         if (LPT > 1 && tid + kTileThreads < ELEMS)                                            \
@@ -127,6 +136,7 @@ __global__ __launch_bounds__(kTileWaves * 64) void exact_tile_kernel(TileArgs a)
         if (tid < SBR / 4) pci = *(const int4 *)(a.cinit + base_ + tid * 4);                  \
     } while (0)
     auto stash_one = [&](int buf, int e, uint4 v) {
+        v = uint4{v.x ^ flip, v.y ^ flip, v.z ^ flip, v.w ^ flip};
         const int r = e / SLOTS, j = e % SLOTS;
         const int p = (j ^ tile_swz(r)) & (SLOTS - 1);
         *(uint4 *)(rows_lds[buf] + r * ROWB + p * 16) = v;
@@ -209,11 +219,11 @@ __global__ __launch_bounds__(kTileWaves * 64) void exact_tile_kernel(TileArgs a)
                             const size_t slot = ((size_t)qi[qb] * a.nchunks + chunk) * 2 + half;
                             if (cnt[qb] < a.E) {
                                 const size_t rec = slot * a.E + cnt[qb];
-                                int4 *dst = (int4 *)(a.rec_val + rec * 16);
+                                uint32_t mask = 0;
 #pragma unroll
-                                for (int g = 0; g < 4; ++g)
-                                    dst[g] = int4{acc[qb][4 * g], acc[qb][4 * g + 1],
-                                                  acc[qb][4 * g + 2], acc[qb][4 * g + 3]};
+                                for (int i = 0; i < 16; ++i)
+                                    mask |= (acc[qb][i] >= V[qb] ? 1u : 0u) << i;
+                                a.rec_mask[rec] = (int32_t)mask;
                                 a.rec_row[rec] = row0;
                             }
                             cnt[qb] += 1;
@@ -268,19 +278,39 @@ __global__ __launch_bounds__(64) void exact_threshold_kernel(const int32_t *bnd,
     if (lane == 0) trig[q] = (int32_t)L;
 }
 
-// Per query: every recorded row scoring >= V, its exact distance and (distance, id) key; sorted;
-// the k smallest written out. Sets flag[q] (and writes nothing else) when records overflowed or
-// the candidates exceed the buffer. Lane l owns slots l, l + 64, l + 128, l + 192 of each group
-// of 256: their counts load together, then record e of all four in one round of loads, so a
-// query costs one memory round trip per record depth rather than per record.
+// The exact distance of a row (bytes ^ flip: shifted values) to a shifted query, over the first
+// dim values: l2 sum (x' - q')^2, ip -(sum (x' + 128)(q' + 128)); integers, exact in int32.
+HM_DEV int32_t exact_int_dist(const uint8_t *row, uint32_t flip, const int8_t *q, int dim,
+                              int metric) {
+    int32_t acc = 0;
+    for (int j0 = 0; j0 < dim; j0 += 16) {
+        const uint4 w = *(const uint4 *)(row + j0);
+        const uint4 u = *(const uint4 *)(q + j0);
+        const uint32_t xs[4] = {w.x ^ flip, w.y ^ flip, w.z ^ flip, w.w ^ flip};
+        const uint32_t qs[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            if (j0 + b >= dim) break;
+            const int xv = (int)(int8_t)(xs[b >> 2] >> (8 * (b & 3)));
+            const int qv = (int)(int8_t)(qs[b >> 2] >> (8 * (b & 3)));
+            if (metric == kL2) acc += (xv - qv) * (xv - qv);
+            else acc += (xv + 128) * (qv + 128);
+        }
+    }
+    return metric == kL2 ? acc : -acc;
+}
+
+// Per query: every recorded row scoring >= V (the record masks), its exact distance recomputed
+// from the row, sorted by (distance, id), the k smallest written out. Sets flag[q] (and writes
+// nothing else) when records overflowed or the candidates exceed the buffer. Lane l owns slots
+// l, l + 64, l + 128, l + 192 of each group of 256: their counts load together, then record e
+// of all four in one round of loads.
 __global__ __launch_bounds__(64) void exact_collect_kernel(CollectArgs a) {
     extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
     uint64_t *ck = (uint64_t *)lds;
     uint32_t *cid = (uint32_t *)(lds + (size_t)a.cap * 8);
     int *ncand = (int *)(lds + (size_t)a.cap * 12);
     const int q = blockIdx.x, lane = lane_id();
-    const int32_t V = a.trig[q];
-    const int32_t qt = a.qterm[q];
     const int slots = a.nchunks * 2;
     const int32_t *cnt_q = a.rec_cnt + (size_t)q * slots;
     if (lane == 0) *ncand = 0;
@@ -303,44 +333,25 @@ __global__ __launch_bounds__(64) void exact_collect_kernel(CollectArgs a) {
             emax = max(emax, c[j]);
         }
         for (int e = 0; e < emax; ++e) {
-            int32_t row0[4];
-            int4 v[4][4];
+            int32_t row0[4], mask[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
+                mask[j] = 0;
                 if (e >= c[j]) continue;
                 const size_t rec = ((size_t)q * slots + base + j * 64 + lane) * a.E + e;
                 row0[j] = a.rec_row[rec];
-                const int4 *vals = (const int4 *)(a.rec_val + rec * 16);
-#pragma unroll
-                for (int g = 0; g < 4; ++g) v[j][g] = vals[g];
+                mask[j] = a.rec_mask[rec];
             }
-#pragma unroll
             for (int j = 0; j < 4; ++j) {
-                if (e >= c[j]) continue;
                 const int half = (base + j * 64 + lane) & 1;
-#pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    const int32_t sv[4] = {v[j][g].x, v[j][g].y, v[j][g].z, v[j][g].w};
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        if (sv[t] < V) continue;
-                        const int64_t row = (int64_t)row0[j] + 8 * g + 4 * half + t;
-                        if (row >= a.n) continue;
-                        int32_t d;
-                        if (a.metric == kL2) {
-                            const int32_t xs = a.xsq[row];
-                            const int32_t acc = sv[t] + ((xs + 1) >> 1);
-                            d = xs + qt - 2 * acc;
-                        } else {
-                            d = -(sv[t] + 128 * qt + 16384 * a.dim);
-                        }
-                        const uint32_t id = a.idmap ? (uint32_t)a.idmap[row] : (uint32_t)row;
-                        const int pos = atomicAdd(ncand, 1);
-                        if (pos < a.cap) {
-                            ck[pos] = key_of_exact(d, a.metric);
-                            cid[pos] = id;
-                        }
-                    }
+                uint32_t m = (uint32_t)mask[j];
+                while (m) {
+                    const int b = __ffs(m) - 1;
+                    m &= m - 1u;
+                    const int64_t row = (int64_t)row0[j] + 8 * (b >> 2) + 4 * half + (b & 3);
+                    if (row >= a.n) continue;
+                    const int pos = atomicAdd(ncand, 1);
+                    if (pos < a.cap) cid[pos] = (uint32_t)row;
                 }
             }
         }
@@ -352,6 +363,15 @@ __global__ __launch_bounds__(64) void exact_collect_kernel(CollectArgs a) {
         if (lane == 0) a.flag[q] = 1;
         return;
     }
+    const int8_t *qrow = a.q8 + (size_t)q * a.qld;
+    for (int t = lane; t < nc; t += 64) {
+        const uint32_t row = cid[t];
+        const int32_t d = exact_int_dist(a.rows.x + (size_t)row * a.rows.ld, a.rows.flip, qrow,
+                                         a.dim, a.metric);
+        ck[t] = key_of_exact(d, a.metric);
+        cid[t] = a.idmap ? (uint32_t)a.idmap[row] : row;
+    }
+    wave_sync();
     int p2 = 1;
     while (p2 < nc) p2 <<= 1;
     bitonic_sort(ck, cid, nc, p2);
@@ -410,20 +430,46 @@ __global__ __launch_bounds__(1024) void exact_subset_ids_kernel(const uint32_t *
     if (t == 1023) *count = part[1023];
 }
 
-// Gathers the subset's rows (dimp bytes each) and per-row terms.
-__global__ void exact_gather_rows_kernel(const int8_t *x8, const int32_t *xsq,
-                                         const int32_t *xsum, const int32_t *idmap, int64_t m,
-                                         int dimp, int8_t *ox8, int32_t *oxsq, int32_t *oxsum) {
+// Gathers the subset's rows into shifted rows of dimp bytes, and their per-row terms.
+__global__ void exact_gather_rows_kernel(I8Rows src, const int32_t *xsq, const int32_t *xsum,
+                                         const int32_t *idmap, int64_t m, int dimp, int8_t *ox8,
+                                         int32_t *oxsq, int32_t *oxsum) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int per = dimp / 16;
     if (e >= m * per) return;
     const int64_t r = e / per;
     const int j = (int)(e % per);
-    const int64_t src = idmap[r];
-    *(uint4 *)(ox8 + r * dimp + j * 16) = *(const uint4 *)(x8 + src * dimp + j * 16);
+    const int64_t from = idmap ? (int64_t)idmap[r] : r;
+    uint4 v = uint4{0, 0, 0, 0};
+    if (j * 16 < src.ld) {
+        v = *(const uint4 *)(src.x + from * src.ld + j * 16);
+        v = uint4{v.x ^ src.flip, v.y ^ src.flip, v.z ^ src.flip, v.w ^ src.flip};
+    }
+    *(uint4 *)(ox8 + r * dimp + j * 16) = v;
     if (j == 0) {
-        oxsq[r] = xsq[src];
-        oxsum[r] = xsum[src];
+        oxsq[r] = xsq[from];
+        oxsum[r] = xsum[from];
+    }
+}
+
+// Per-row terms over the first dim values of 8-bit rows: one wave per row, four rows per block.
+__global__ __launch_bounds__(256) void exact_row_terms_kernel(I8Rows src, int64_t n, int dim,
+                                                              int32_t *xsq, int32_t *xsum) {
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    const int lane = lane_id();
+    int32_t s2 = 0, s1 = 0;
+    for (int i = lane; i < dim; i += 64) {
+        const uint8_t b = src.x[r * src.ld + i] ^ (uint8_t)(src.flip & 0xff);
+        const int v = (int)(int8_t)b;
+        s2 += v * v;
+        s1 += v;
+    }
+    s2 = wave_sum(s2);
+    s1 = wave_sum(s1);
+    if (lane == 0) {
+        xsq[r] = s2;
+        xsum[r] = s1;
     }
 }
 
@@ -499,12 +545,19 @@ hipError_t subset_ids_launch(const uint32_t *bits, int64_t n, int32_t *idmap, in
     return hipGetLastError();
 }
 
-hipError_t gather_rows_launch(const int8_t *x8, const int32_t *xsq, const int32_t *xsum,
+hipError_t gather_rows_launch(I8Rows src, const int32_t *xsq, const int32_t *xsum,
                               const int32_t *idmap, int64_t m, int dimp, int8_t *ox8,
                               int32_t *oxsq, int32_t *oxsum, hipStream_t s) {
     const int64_t total = m * (dimp / 16);
     hipLaunchKernelGGL(exact_gather_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256),
-                       0, s, x8, xsq, xsum, idmap, m, dimp, ox8, oxsq, oxsum);
+                       0, s, src, xsq, xsum, idmap, m, dimp, ox8, oxsq, oxsum);
+    return hipGetLastError();
+}
+
+hipError_t row_terms_launch(I8Rows src, int64_t n, int dim, int32_t *xsq, int32_t *xsum,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(exact_row_terms_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, src,
+                       n, dim, xsq, xsum);
     return hipGetLastError();
 }
 

@@ -9,11 +9,29 @@ namespace hm {
 constexpr int kTileRows = 256;     // chunks are multiples of this many rows (the largest superblock)
 constexpr int kTileWaves = 8;      // waves per work-group: two per SIMD
 constexpr int32_t kTilePadInit = -(1 << 30);
+// A tile plan over m rows has at most min(kTileMaxChunks, max(1, m / 4096)) chunks, each rounded
+// up to a multiple of kTileRows: the range search reads rows (and accumulator starts) up to
+// m + tile_pad_rows(m).
+constexpr int kTileMaxChunks = 1024;
+constexpr int64_t kTilePadRows = (int64_t)kTileMaxChunks * kTileRows;
+inline int64_t tile_pad_rows(int64_t m) {
+    const int64_t chunks = m / 4096 + 1;
+    return (chunks < kTileMaxChunks ? chunks : kTileMaxChunks) * kTileRows;
+}
+
+// Rows of 8-bit values as the matrix cores read them: byte ^ flip is the shifted value v - 128
+// as int8 (flip 0x80 per byte over raw 0..255 bytes, 0 over bytes already shifted). Bytes at
+// and beyond the row width (and rows at and beyond n) read as zero.
+struct I8Rows {
+    const uint8_t *x;
+    int64_t ld;      // row stride in bytes, a multiple of 16
+    uint32_t flip;   // 0x80808080 or 0
+};
 
 struct TileArgs {
-    const int8_t *x8;      // (nchunks * chunk, 32 * NCH) shifted rows, zero padding
-    const int32_t *cinit;  // (nchunks * chunk) accumulator start per row
-    const int8_t *q8;      // (nq, 32 * NCH) shifted queries
+    I8Rows rows;           // ld == 32 * NCH; readable to the end of the last chunk
+    const int32_t *cinit;  // accumulator start per row (kTilePadInit beyond the valid rows)
+    const int8_t *q8;      // (nq, 32 * NCH) shifted queries, zero padded
     int nq;
     int64_t chunk;  // rows per chunk, a multiple of kTileRows
     int nchunks;
@@ -21,16 +39,18 @@ struct TileArgs {
     int32_t *bnd;      // bound pass out: (nq, nchunks, 2, T)
     const int32_t *trig;  // range pass: (nq) V
     int E;                // records per (query, chunk, half)
-    int32_t *rec_val;     // (nq, nchunks, 2, E, 16) scores
+    int32_t *rec_mask;    // (nq, nchunks, 2, E) which of the lane's 16 rows reach V (bit 4g+t:
+                          // row 8g + 4 (lane / 32) + t of the block)
     int32_t *rec_row;     // (nq, nchunks, 2, E) first row of the 32-row block
     int32_t *rec_cnt;     // (nq, nchunks, 2) records written (> E: overflow)
 };
 
 struct CollectArgs {
-    const int32_t *rec_val, *rec_row, *rec_cnt, *trig;
-    const int32_t *xsq;    // l2: |x'|^2 per row
+    const int32_t *rec_mask, *rec_row, *rec_cnt;
+    I8Rows rows;           // the rows scanned (exact distances are recomputed from them)
+    const int8_t *q8;      // shifted queries of this launch, row stride qld bytes
+    int qld;
     const int32_t *idmap;  // row -> id, or null (identity)
-    const int32_t *qterm;  // per query: |q'|^2 (l2) or sum(q') (ip)
     int64_t n;             // valid rows
     int nchunks, E, k, metric, dim, cap;
     int64_t *ids;
@@ -45,14 +65,19 @@ hipError_t tile_launch(int mode, int nch, int qb, int T, const TileArgs &a, dim3
 hipError_t threshold_launch(const int32_t *bnd, int per_query, int k, int32_t *trig, int nq,
                             hipStream_t s);
 hipError_t collect_launch(const CollectArgs &a, int nq, hipStream_t s);
+// Accumulator starts of rows [0, n) and kTilePadInit for [n, rows).
 hipError_t cinit_launch(const int32_t *xsq, const int32_t *xsum, int64_t n, int64_t rows,
                         int metric, int32_t *cinit, hipStream_t s);
 hipError_t subset_ids_launch(const uint32_t *bits, int64_t n, int32_t *idmap, int64_t *count,
                              hipStream_t s);
-hipError_t gather_rows_launch(const int8_t *x8, const int32_t *xsq, const int32_t *xsum,
+// Gathers the subset's rows (idmap null: rows 0..m-1) into shifted int8 rows of dimp bytes, and
+// their per-row terms.
+hipError_t gather_rows_launch(I8Rows src, const int32_t *xsq, const int32_t *xsum,
                               const int32_t *idmap, int64_t m, int dimp, int8_t *ox8,
                               int32_t *oxsq, int32_t *oxsum, hipStream_t s);
-
+// Per-row terms of 8-bit rows over their first dim values: sum (v - 128)^2 and sum (v - 128).
+hipError_t row_terms_launch(I8Rows src, int64_t n, int dim, int32_t *xsq, int32_t *xsum,
+                            hipStream_t s);
 
 // ---------------------------------------------------------------- float data (exact_ftile.hip)
 

@@ -26,6 +26,9 @@ struct hm_index {
     hm::DevBuf<int32_t> links0, links_up, upper;
     hm::DevBuf<float> vec;        // (n, dim)
     hm::DevBuf<uint8_t> u8, codes;  // (n, dimp)
+    // exact search over u8 (set at open): per row sum (v - 128)^2, sum (v - 128), and the range
+    // search's accumulator start for this index's metric
+    hm::DevBuf<int32_t> ex_xsq, ex_xsum, ex_cinit;
     // search scratch
     hm::DevBuf<uint32_t> bits, vlist;
     hm::DevBuf<uint64_t> spill_key;
@@ -65,6 +68,9 @@ struct hm_index {
         for (auto *b : i32) b->release();
         vec.release();
         u8.release();
+        ex_xsq.release();
+        ex_xsum.release();
+        ex_cinit.release();
         codes.release();
         bits.release();
         vlist.release();

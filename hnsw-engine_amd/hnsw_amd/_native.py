@@ -33,6 +33,7 @@ SYMBOLS = (
     "hm_exact_timing",
     "hm_set_exact_tuning",
     "hm_index_exact",
+    "hm_index_exact_dev",
     "hm_build",
     "hm_model_size",
     "hm_set_stop_model",
@@ -133,6 +134,7 @@ def lib() -> ctypes.CDLL:
             "hm_exact_timing": ([vp], ctypes.c_int),
             "hm_set_exact_tuning": ([i32, i32], ctypes.c_int),
             "hm_index_exact": ([vp, vp, i64, i32, vp, vp, vp], ctypes.c_int),
+            "hm_index_exact_dev": ([vp, vp, i64, i32, vp, ctypes.c_int, vp, vp], ctypes.c_int),
             "hm_build": (
                 [i64, i64, i32, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp],
                 ctypes.c_int,

@@ -190,8 +190,10 @@ int hm_exact(const float *data, int64_t n, int32_t dim, int32_t metric, const fl
  * d_queries float32 (nq, dim), d_allowed_bits ceil(n/32) uint32 words (bit i of word i/32 set:
  * row i allowed) or null, outputs int64 / float64 (nq, k). exact_u8 = 1 selects the int8
  * matrix-core path: the caller guarantees that every data and query value is an integer in
- * 0..255 (metric l2 or ip); 0 selects the float64 path. Scratch is allocated per call and the
- * call synchronises `stream` (a hipStream_t, null: the default stream) before it returns. */
+ * 0..255 (metric l2 or ip); 0 selects the float path (float32 matrix-core shortlist, float64
+ * ranking). The int8 path converts the rows per call into scratch that it keeps per device
+ * between calls (calls on one device take turns); the call synchronises `stream` (a
+ * hipStream_t, null: the default stream) before it returns. */
 int hm_exact_dev(const float *d_data, int64_t n, int32_t dim, int32_t metric,
                  const float *d_queries, int64_t nq, int32_t k, const uint32_t *d_allowed_bits,
                  int exact_u8, int64_t *d_ids, double *d_dists, void *stream);
@@ -202,6 +204,15 @@ int hm_exact_dev(const float *d_data, int64_t n, int32_t dim, int32_t metric,
  * float64 (nq, k). Same results as hm_exact on the index's vectors. */
 int hm_index_exact(hm_index *index, const float *queries, int64_t nq, int32_t k,
                    const uint8_t *allowed, int64_t *ids, double *dists);
+
+/* hm_index_exact with queries, mask and outputs in HBM (the layouts of hm_exact_dev), on the
+ * index's stream, synchronised before return (the reference's oracle.exact_search run over an
+ * index's vectors, oracle.py:35-126). An index of 0..255 integer rows keeps them as bytes with
+ * their row terms, so the int8 path reads them in place: queries_u8 = 1 declares every query
+ * value an integer in 0..255. */
+int hm_index_exact_dev(hm_index *index, const float *d_queries, int64_t nq, int32_t k,
+                       const uint32_t *d_allowed_bits, int queries_u8, int64_t *d_ids,
+                       double *d_dists);
 
 /* Tile shape of the range searches (no counterpart in the reference; 0 restores the automatic
  * choice), process-wide: query_blocks (1, 2 or 4) blocks of 32 queries per wave of the int8

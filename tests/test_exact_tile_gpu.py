@@ -5,7 +5,7 @@ k <= 128. Bar: ids and float64 distances bit for bit. The cases cover every tile
 launcher picks (dim 16 / 40 / 128 / 200 -> 1, 2, 4, 8 K-steps of 32; 100, 300 and 3000 queries
 -> 1, 2 and 4 query blocks per wave), both metrics, masks before and after compaction, and
 tie-heavy data whose records overflow, so that the flagged queries are re-run on the chunked
-kernels.
+kernels; and an index's own byte rows read in place (hm_index_exact / hm_index_exact_dev).
 """
 
 from __future__ import annotations
@@ -102,3 +102,52 @@ def test_exact_dev_matches_host_entry_point() -> None:
         ctypes.c_void_p(stream.cuda_stream)))
     np.testing.assert_array_equal(ids.cpu().numpy(), want_ids)
     np.testing.assert_array_equal(d.cpu().numpy(), want_d)
+
+
+def _u8_engine(data, metric="l2"):
+    """An index over the rows (graph degree kept small: only its exact search is used)."""
+    from hnsw_amd.engine import MI355XEngine
+
+    n = data.shape[0]
+    levels = np.floor(-np.log(1.0 - np.random.default_rng(1).random(n)) / np.log(4)).astype(np.int64)
+    eng = MI355XEngine(data.shape[1], metric, 4, 8, "heuristic", False, False)
+    eng.insert(data, levels)
+    return eng
+
+
+@pytest.mark.parametrize("dim,metric", [(128, "l2"), (100, "l2"), (100, "ip"), (40, "l2")])
+def test_index_rows_read_in_place_match_oracle(dim: int, metric: str) -> None:
+    """The index keeps raw 0..255 bytes (rows padded to 16 bytes); the range search reads them
+    in place (byte ^ 0x80 = the shifted value) with row terms computed at open. dim 100: the
+    rows are 112 bytes and the tile reads 128, the last 16 as zeros."""
+    data, queries = clustered_u8(40_000, dim, 500, n_clusters=40, seed=16 + dim)
+    eng = _u8_engine(data, metric)
+    for allowed in (None, np.random.default_rng(17).random(40_000) < 0.6):
+        print(f"case index exact dim={dim} {metric} masked={allowed is not None}", flush=True)
+        ids, d = eng.exact_search(queries, 10, allowed)
+        info = last_range_search()
+        print(info, flush=True)
+        assert info["range_ms"] > 0
+        t_ids, t_d = oracle.exact_search(data, queries, 10, metric, allowed=allowed, threads=8)
+        np.testing.assert_array_equal(ids, t_ids)
+        np.testing.assert_array_equal(d, t_d)
+
+
+def test_index_exact_dev_matches_host_entry_point() -> None:
+    import torch
+
+    data, queries = clustered_u8(30_000, 128, 300, n_clusters=30, seed=18)
+    eng = _u8_engine(data)
+    k = 10
+    want_ids, want_d = eng.exact_search(queries, k)
+    dev = torch.device("cuda:0")
+    q = torch.from_numpy(queries).to(dev)
+    ids = torch.empty((queries.shape[0], k), dtype=torch.int64, device=dev)
+    d = torch.empty((queries.shape[0], k), dtype=torch.float64, device=dev)
+    print("case index_exact_dev", flush=True)
+    for u8 in (1, 0):  # the int8 path, then the float path on the same index
+        _native.check(_native.lib().hm_index_exact_dev(
+            eng._device(), ctypes.c_void_p(q.data_ptr()), queries.shape[0], k, None, u8,
+            ctypes.c_void_p(ids.data_ptr()), ctypes.c_void_p(d.data_ptr())))
+        np.testing.assert_array_equal(ids.cpu().numpy(), want_ids)
+        np.testing.assert_array_equal(d.cpu().numpy(), want_d)
