@@ -260,6 +260,28 @@ def main() -> int:
     assert np.array_equal(ids_dev.cpu().numpy(), chosen["ids"]), "graph replay differs"
     graph_info = {"ms_per_step": graph_ms, "qps": nq / (graph_ms / 1e3)}
 
+    # Host call to host results: the same batch through hm_search (queries copied in over PCIe,
+    # ids and distances copied out), median of 3 calls.
+    h_ids = np.empty((nq, k), dtype=np.int64)
+    h_d = np.empty((nq, k), dtype=np.float64)
+    hopts = _native.SearchOpts()
+    hopts.nq, hopts.k_all, hopts.ef_all, hopts.kstride, hopts.mask_all = nq, k, ef, k, -1
+    hopts.learned = int(use_learned)
+
+    def host_call() -> float:
+        t = time.perf_counter()
+        _native.check(lib.hm_search(handle, _native.ptr(queries), ctypes.byref(hopts),
+                                    _native.ptr(h_ids), _native.ptr(h_d), None, None))
+        return time.perf_counter() - t
+
+    host_call()
+    host_times = sorted(host_call() for _ in range(3))
+    host_info = {"qps": nq / host_times[1], "ms_per_batch": host_times[1] * 1e3,
+                 "ids_match_device_path": bool(np.array_equal(h_ids, chosen["ids"])),
+                 "note": "hm_search with host buffers: H2D queries + search + D2H ids/dists, "
+                         "median of 3"}
+    log(f"[bench] host-buffer (PCIe-inclusive) batch: {host_info}")
+
     # Algorithmic bytes of one launch: every distance reads one stored row, every expansion one
     # links row (upper-layer rows counted at the layer-0 width), as the reference counts them.
     row_bytes = eng.dim if exact_u8 else eng.dim * 4
@@ -268,12 +290,17 @@ def main() -> int:
     link_bytes = ((2 * eng.M + 15) // 16 * 16) * 4
     algo_bytes = float(chosen["dc"].sum() * row_bytes + chosen["hops"].sum() * link_bytes)
     achieved = algo_bytes / (launch_ms / 1e3) / 1e9
-    traffic = None
+    # HBM traffic comes from rocprofv3 counters, which this process cannot read: the committed
+    # profile of the same launch (same ef and variant) supplies it, labelled with its commit.
+    traffic, traffic_source = None, None
     tfile = ROOT / "profiles" / f"search_{args.workload}_traffic.json"
     if tfile.is_file():
         tj = json.loads(tfile.read_text())
         if tj.get("ef") == ef and bool(tj.get("learned", False)) == bool(use_learned):
             traffic = tj.get("hbm_bytes_per_launch")
+            traffic_source = (f"profiles/{tfile.name}: rocprofv3 FETCH_SIZE + WRITE_SIZE per "
+                              f"dispatch x copy calibration, measured at commit "
+                              f"{tj.get('commit', 'unrecorded')}")
 
     cpu = None
     if not args.no_cpu:
@@ -310,10 +337,15 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
             "kernel": "one batch search: descend_kernel, order_by_entry_kernel, layer-0 kernel",
             "launch_ms": launch_ms, "algo_bytes": algo_bytes,
+            "bytes_unit": (f"{row_bytes} B per distance (the "
+                           + ("uint8 copy of the row, exact for these integer values"
+                              if exact_u8 else "float32 row") + f"), {link_bytes} B per expansion "
+                           "(one layer-0 link row)"),
         },
+        "host_buffers": host_info,
         "oracle": oracle_info,
         "cpu_baseline": cpu,
     }
