@@ -68,7 +68,7 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--train-queries", type=int, default=10_000)
     ap.add_argument("--learned-ef", default="56,64")
-    ap.add_argument("--thresholds", default="0.6,0.7,0.8,0.85,0.9,0.95")
+    ap.add_argument("--thresholds", default="0.5,0.6,0.7,0.8,0.85,0.9,0.95")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB on the device three times (run under rocprofv3 --pmc "
                          "FETCH_SIZE / WRITE_SIZE: 1,048,576 KiB over the counter's KiB per "
@@ -158,29 +158,40 @@ def main() -> int:
         _, t_d = exact_search(data, train_q, k)
         thr = lrn.recall_thresholds(t_d, k)
         xs, ys = [], []
-        for ef_l in [int(v) for v in args.learned_ef.split(",")]:
+        efs_l = [int(v) for v in args.learned_ef.split(",")]
+        for ef_l in efs_l:
             trace, length, final = eng.trace(train_q, k, ef_l, thr, steps=256)
-            x, y = lrn.dataset_from_traces(trace, length, final)
+            # Rows of long queries weigh more: they set the batch time.
+            x, y = lrn.dataset_from_traces(trace, length, final, max_rows=200_000 // len(efs_l),
+                                           seed=ef_l, long_weight=2.0)
             xs.append(x)
             ys.append(y)
         x = np.concatenate(xs)
         y = np.concatenate(ys)
-        if x.shape[0] > 200_000:
-            pick = np.sort(np.random.default_rng(1).choice(x.shape[0], 200_000, replace=False))
-            x, y = x[pick], y[pick]
         params, losses = lrn.train(x, y, epochs=12, seed=3)
         t_train = time.perf_counter() - t0
         log(f"[bench] stop model: {x.shape[0]} rows ({y.mean():.2f} positive), loss "
             f"{losses[0]:.4f} -> {losses[-1]:.4f}, {t_train:.1f}s")
+        # Expansion caps: the tail-aware oracle's cap on the training traces (largest ef), and
+        # looser ones; 0 is no cap.
+        slots = 16 * torch.cuda.get_device_properties(0).multi_processor_count
+        t_train = lrn.oracle_stop_tail(trace, length, k, args.target_recall, slots, length)
+        c0 = int(t_train["cap"]) if t_train else 0
+        caps = sorted({0, c0, int(1.25 * c0), int(1.5 * c0)} if c0 else {0})
+        log(f"[bench] stop caps from the training traces: {caps}")
         lsweep = []
-        for ef_l in [int(v) for v in args.learned_ef.split(",")]:
-            for tau in [float(v) for v in args.thresholds.split(",")]:
-                eng.set_stop_model(params, tau)
-                r = measure(ef_l, learned=True)
-                r["threshold"] = tau
-                lsweep.append(r)
-                log(f"[bench] learned ef={ef_l} threshold={tau} recall={r['recall']:.4f} "
-                    f"qps={r['qps']:,.0f} dist/q={r['dcount_mean']:.0f} hops/q={r['hops_mean']:.0f}")
+        for cap in caps:
+            eng.set_stop_cap(cap)
+            for ef_l in [int(v) for v in args.learned_ef.split(",")]:
+                for tau in [float(v) for v in args.thresholds.split(",")]:
+                    eng.set_stop_model(params, tau)
+                    r = measure(ef_l, learned=True)
+                    r["threshold"] = tau
+                    r["cap"] = cap
+                    lsweep.append(r)
+                    log(f"[bench] learned ef={ef_l} threshold={tau} cap={cap} "
+                        f"recall={r['recall']:.4f} qps={r['qps']:,.0f} "
+                        f"dist/q={r['dcount_mean']:.0f} hops/q={r['hops_mean']:.0f}")
         # Ceiling of any stop rule on this beam: the oracle stopping points of the measured
         # queries (known future) at the target recall, against the plain beam's expansions.
         thr_m = lrn.recall_thresholds(true_d, k)
@@ -195,6 +206,12 @@ def main() -> int:
                            "max_speedup_in_expansions":
                                float(np.minimum(len_p, 256).mean()) / ceiling[0]}
             log(f"[bench] stop ceiling: {ceiling_info}")
+        # The same ceiling for the batch time: list scheduling of the per-query layer-0
+        # expansions on the launch's wave slots (16 per compute unit), with stopping points
+        # chosen to minimise that makespan.
+        slots = 16 * torch.cuda.get_device_properties(0).multi_processor_count
+        tail = lrn.oracle_stop_tail(tr_o, len_o, k, args.target_recall, slots, len_p)
+        log(f"[bench] tail-aware stop ceiling: {tail}")
         ok = [r for r in lsweep if r["recall"] >= args.target_recall]
         best = max(ok, key=lambda r: r["qps"]) if ok else None
         learned_best = best
@@ -203,14 +220,17 @@ def main() -> int:
             "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
             "train_seconds": t_train,
             "oracle_stop_ceiling": ceiling_info,
-            "sweep": [{kk: r[kk] for kk in ("ef", "threshold", "recall", "qps", "dcount_mean",
-                                             "hops_mean")} for r in lsweep],
+            "oracle_stop_ceiling_tail_aware": tail,
+            "sweep": [{kk: r[kk] for kk in ("ef", "threshold", "cap", "recall", "qps",
+                                             "dcount_mean", "hops_mean")} for r in lsweep],
         }
         if best is not None and best["qps"] > plain["qps"]:
             chosen = best
             eng.set_stop_model(params, best["threshold"])
+            eng.set_stop_cap(best["cap"])
         learned_info["best"] = None if learned_best is None else {
-            kk: learned_best[kk] for kk in ("ef", "threshold", "recall", "qps", "dcount_mean")}
+            kk: learned_best[kk] for kk in ("ef", "threshold", "cap", "recall", "qps",
+                                            "dcount_mean")}
     ef = chosen["ef"]
     use_learned = chosen.get("learned", False)
 
@@ -324,8 +344,8 @@ def main() -> int:
             "workload": f"{args.workload}: {w.n} x {w.dim} float32, integer values 0..255, "
             f"{w.n_clusters} clusters, {nq} queries, L2, M={w.M}, efC={w.ef_construction}",
             "k": k, "ef": ef, "recall": chosen["recall"],
-            "variant": (f"learned stop (threshold {chosen['threshold']})" if use_learned
-                        else "plain beam"),
+            "variant": (f"learned stop (threshold {chosen['threshold']}, cap {chosen['cap']})"
+                        if use_learned else "plain beam"),
             "plain": {"ef": plain["ef"], "recall": plain["recall"], "qps": plain["qps"]},
             "learned_speedup_vs_plain": (
                 learned_info["best"]["qps"] / plain["qps"]

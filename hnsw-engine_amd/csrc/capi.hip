@@ -226,7 +226,7 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap,
         a.stop_w = h->stop_w.p;
         a.stop_logit = h->stop_logit;
         a.stop_tmin = h->stop_tmin;
-        a.lds_per_wave += kModelLdsBytes;
+        a.stop_tmax = h->stop_tmax;
     }
     if (a.lds_per_wave > 160 * 1024) return invalid("ef too large for the on-chip search state");
     int slots = search_slots(mode, filt, a, a.lds_per_wave, h->cus);
@@ -765,7 +765,7 @@ std::vector<int64_t> graph_key(const hm_index *h, const float *d_queries, const 
     return {P(d_queries), o->nq, o->k_all, o->ef_all, o->kstride, o->mask_all, P(o->mask_index),
             o->learned, P(d_mask_bits), exact_u8, P(b.ids), P(b.dists), P(b.dc), P(b.hops),
             P(b.fail), h->tune_fast_hlog, h->tune_hlog, h->tune_waves_per_cu, h->tune_probe_limit,
-            h->tune_schedule, h->tune_descent_cap, h->stop_on, logit_bits, h->stop_tmin,
+            h->tune_schedule, h->tune_descent_cap, h->stop_on, logit_bits, h->stop_tmin, h->stop_tmax,
             P(h->stop_w.p), P(h->bits.p), (int64_t)h->bits.cap, P(h->vlist.p),
             (int64_t)h->vlist.cap, P(h->spill_key.p), (int64_t)h->spill_key.cap,
             P(h->spill_id.p), (int64_t)h->spill_id.cap, P(h->queue.p), P(h->stats.p),

@@ -48,7 +48,7 @@ struct hm_index {
     // learned stop (hm_set_stop_model)
     hm::DevBuf<float> stop_w;
     float stop_logit = 0.f;
-    int stop_tmin = 0;
+    int stop_tmin = 0, stop_tmax = 0;
     bool stop_on = false;
     // tuning (0: automatic): visited-hash size of the register-beam and general kernels, waves
     // per compute unit

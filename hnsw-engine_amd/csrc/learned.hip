@@ -280,4 +280,11 @@ int hm_set_stop_model(hm_index *h, const float *params, float threshold, int32_t
     return HM_OK;
 }
 
+int hm_set_stop_cap(hm_index *h, int32_t max_steps) {
+    if (!h) return invalid("null index");
+    if (max_steps < 0) return invalid("max_steps must be >= 0");
+    h->stop_tmax = max_steps;
+    return HM_OK;
+}
+
 }  // extern "C"

@@ -43,7 +43,6 @@ struct Lds {
     int32_t *nbacc;
     uint32_t *nbal;
     uint32_t *hash;
-    uint32_t *model;  // learned stop: the packed model, kModelWords words per lane
     uint32_t *hist;  // 256 bins for the refill's radix select
     uint64_t *misc;  // [0..1]: first spilled entry
     // general register beam: R (64 nr entries: key, id, expanded flag) and C (64 nc entries)
@@ -91,7 +90,7 @@ HM_DEV Lds carve(uint8_t *base, const SearchArgs &a, int mode) {
 }
 
 // Compact layout of the register-beam kernel (fields it does not use stay null).
-HM_DEV Lds carve_fast(uint8_t *base, const SearchArgs &a, int mode, bool model) {
+HM_DEV Lds carve_fast(uint8_t *base, const SearchArgs &a, int mode) {
     auto up = [](int v) { return (v + 15) & ~15; };
     Lds L = {};
     uint8_t *p = base;
@@ -106,10 +105,6 @@ HM_DEV Lds carve_fast(uint8_t *base, const SearchArgs &a, int mode, bool model) 
     // stages R in rk[0] / ri[0] / nbal, so they share that space.
     L.nbi = L.ri[0];
     L.nbacc = (int32_t *)L.nbal;
-    if (model) {
-        L.model = (uint32_t *)p;
-        p += kModelLdsBytes;
-    }
     L.hash = (uint32_t *)p;
     return L;
 }
@@ -988,11 +983,12 @@ struct StopState {
 // The key of the candidate the search would expand next (the first unexpanded entry of R, else
 // the head of the tie list); the worst key of R when none is left.
 
-// The model as the search kernel holds it: per lane (hidden unit) kModelWords words in the
-// wave's LDS, word i at w[i * 64 + lane]: words 0..7 the W1 column as bfloat16 pairs, word 8 b1
-// (low half) and W2 (high half).
+// The model as the search kernel holds it: per lane (hidden unit) kModelWords words in
+// registers for the whole launch: words 0..7 the W1 column as bfloat16 pairs, word 8 b1 (low
+// half) and W2 (high half). Registers rather than LDS keep the stop kernel at the plain
+// kernel's LDS footprint, so as many waves fit per compute unit.
 struct StopModel {
-    const uint32_t *w;
+    uint32_t w[kModelWords];
     float b2;
 };
 
@@ -1000,27 +996,20 @@ HM_DEV uint32_t bf16_bits(float x) {  // the value is already a bfloat16: keep i
     return __float_as_uint(x) >> 16;
 }
 
-HM_DEV void load_model(const SearchArgs &a, const Lds &L, StopModel &M) {
+HM_DEV void load_model(const SearchArgs &a, StopModel &M) {
     const int lane = lane_id();
 #pragma unroll
     for (int i = 0; i < kFeatures; i += 2)
-        L.model[(i / 2) * 64 + lane] =
-            bf16_bits(a.stop_w[kModelW1 + i * kHidden + lane]) |
-            (bf16_bits(a.stop_w[kModelW1 + (i + 1) * kHidden + lane]) << 16);
-    L.model[8 * 64 + lane] =
-        bf16_bits(a.stop_w[kModelB1 + lane]) | (bf16_bits(a.stop_w[kModelW2 + lane]) << 16);
-    M.w = L.model;
+        M.w[i / 2] = bf16_bits(a.stop_w[kModelW1 + i * kHidden + lane]) |
+                     (bf16_bits(a.stop_w[kModelW1 + (i + 1) * kHidden + lane]) << 16);
+    M.w[8] = bf16_bits(a.stop_w[kModelB1 + lane]) | (bf16_bits(a.stop_w[kModelW2 + lane]) << 16);
     M.b2 = a.stop_w[kModelB2];
-    wave_sync();
 }
 
 // 16 -> 64 (ReLU) -> 1 with float32 accumulation on normalised features (mean and inverse
 // standard deviation from the model, uniform across lanes); lane j owns hidden unit j.
 HM_DEV float model_logit(const SearchArgs &a, const StopModel &M, const float *f) {
-    const int lane = lane_id();
-    uint32_t w[kModelWords];
-#pragma unroll
-    for (int i = 0; i < kModelWords; ++i) w[i] = M.w[i * 64 + lane];
+    const uint32_t *w = M.w;
     float z = __uint_as_float(w[8] << 16);
 #pragma unroll
     for (int i = 0; i < kFeatures; i += 2) {
@@ -1244,6 +1233,7 @@ HM_DEV bool layer0_fast(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
             // The learned stop evaluates (and so computes features) every eighth expansion once
             // stop_tmin expansions ran (every fourth / second: about 3 / 8 % fewer queries/s at the
             // chosen setting); traces record every expansion.
+            if (AUX == 2 && a.stop_tmax > 0 && S.t >= a.stop_tmax) break;
             if (AUX == 2 && (S.t < a.stop_tmin || (S.t & 7) != 0)) continue;
             const uint64_t valid_now = rn >= 64 ? ~0ull : ((1ull << rn) - 1ull);
             const uint64_t live_now = valid_now & ~expd;
@@ -1496,6 +1486,7 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
             const uint64_t kth = readlane_u64(rk, kk - 1);
             S.since = kth == S.last_kth ? S.since + 1 : 0;
             S.last_kth = kth;
+            if (AUX == 2 && a.stop_tmax > 0 && S.t >= a.stop_tmax) break;
             if (AUX == 2 && (S.t < a.stop_tmin || (S.t & 7) != 0)) continue;
             const uint64_t valid_now = rn >= 64 ? ~0ull : ((1ull << rn) - 1ull);
             const uint64_t live_now = valid_now & ~expd[0];
@@ -2147,7 +2138,7 @@ HM_DEV void search_loop(const SearchArgs &a) {
     extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
     const int slot = blockIdx.x;
     const Lds L = BNR > 0 ? carve_beam(lds_raw, a, MODE)
-                  : FAST  ? carve_fast(lds_raw, a, MODE, AUX == 2)
+                  : FAST  ? carve_fast(lds_raw, a, MODE)
                           : carve(lds_raw, a, MODE);
     Visited V;
     V.bits = a.bitsets + (size_t)slot * a.bitset_words;
@@ -2164,7 +2155,7 @@ HM_DEV void search_loop(const SearchArgs &a) {
     V.hlog = a.h_log2;
     V.count = 0;
     StopModel M = {};
-    if constexpr (AUX == 2) load_model(a, L, M);
+    if constexpr (AUX == 2) load_model(a, M);
     for (;;) {
         int pos = 0;
         if (lane_id() == 0) pos = atomicAdd(a.queue, 1);
@@ -2196,7 +2187,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void se
 // Register-beam kernel with the learned stop (its model in LDS): three waves per SIMD
 // (<= 168 VGPRs), the most its LDS allows with 4096 visited slots.
 template <int MODE, int SH>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void search_stop_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SH == kShapeM16D128 ? 4 : 3))) void search_stop_kernel(
     SearchArgs a) {
     search_loop<MODE, false, true, 2, SH>(a);
 }

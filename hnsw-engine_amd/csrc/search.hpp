@@ -80,10 +80,11 @@ struct SearchArgs {
     // kernel classes launches each class over its own subset.
     const int32_t *subset;
     // learned stopping (register-beam kernel only): model parameters (layout below) and the
-    // logit above which a beam stops; at least stop_tmin layer-0 expansions run first
+    // logit above which a beam stops; at least stop_tmin layer-0 expansions run first, at most
+    // stop_tmax (0: no cap)
     const float *stop_w;
     float stop_logit;
-    int stop_tmin;
+    int stop_tmin, stop_tmax;
     // traces of the plain register-beam search: per query up to trace_steps rows of
     // kTraceCols floats, the number of expansions and the final tie-aware hit count
     float *trace;
@@ -114,7 +115,6 @@ constexpr int kModelW2 = kModelB1 + kHidden;
 constexpr int kModelB2 = kModelW2 + kHidden;
 constexpr int kModelSize = kModelB2 + 1;
 constexpr int kModelWords = kFeatures / 2 + 1;        // packed bfloat16 words per hidden unit
-constexpr int kModelLdsBytes = kModelWords * 64 * 4;  // the search kernel's LDS copy
 constexpr int kTraceCols = kFeatures + 2;  // features, hits, |R|
 
 // Bytes of the visited hash with 1 << h_log2 slots: 16-bit quotient slots when the ids have at

@@ -37,6 +37,7 @@ SYMBOLS = (
     "hm_build",
     "hm_model_size",
     "hm_set_stop_model",
+    "hm_set_stop_cap",
     "hm_trace_dev",
     "hm_trace",
     "hm_mlp_grad",
@@ -141,6 +142,7 @@ def lib() -> ctypes.CDLL:
             ),
             "hm_model_size": ([], ctypes.c_int),
             "hm_set_stop_model": ([vp, vp, ctypes.c_float, i32], ctypes.c_int),
+            "hm_set_stop_cap": ([vp, i32], ctypes.c_int),
             "hm_trace_dev": ([vp, vp, i64, i32, i32, ctypes.c_int, vp, vp, i32, vp, vp, vp], ctypes.c_int),
             "hm_trace": ([vp, vp, i64, i32, i32, vp, vp, i32, vp, vp], ctypes.c_int),
             "hm_mlp_grad": ([vp, vp, i64, vp, ctypes.c_float, vp, vp], ctypes.c_int),

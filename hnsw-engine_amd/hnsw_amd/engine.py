@@ -261,6 +261,10 @@ class MI355XEngine:
                                                       int(min_steps)))
         self._stop = (p, float(threshold), int(min_steps))
 
+    def set_stop_cap(self, max_steps: int) -> None:
+        """Cap stopped beams at ``max_steps`` layer-0 expansions (0: no cap)."""
+        _native.check(_native.lib().hm_set_stop_cap(self._device(), int(max_steps)))
+
     def exact_search(self, queries: np.ndarray, k: int, allowed: np.ndarray | None = None):
         """Exact k-NN (ids, float64 distances) over the vectors in HBM, optionally restricted to
         the allowed rows (the masked brute force, ``benchmarks/run_filtered.py:76-106``)."""

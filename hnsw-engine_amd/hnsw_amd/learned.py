@@ -45,20 +45,48 @@ def recall_thresholds(true_dist: np.ndarray, k: int) -> np.ndarray:
 
 
 def dataset_from_traces(trace: np.ndarray, length: np.ndarray, final: np.ndarray,
-                        max_rows: int | None = None, seed: int = 0):
+                        max_rows: int | None = None, seed: int = 0, long_weight: float = 0.0):
     """``(x, y)``: one row per recorded expansion; label 1 when the hits already equal the
-    final hits. With ``max_rows`` a uniform sample (fixed seed) of that many rows."""
+    final hits. With ``max_rows`` a sample (fixed seed) of that many rows: uniform, or with
+    ``long_weight`` > 0 drawn with probability proportional to (query length / mean
+    length) ** long_weight, so that the long queries, which set a batch's time under the
+    longest-first schedule, weigh more in training."""
     steps = trace.shape[1]
+    ln = np.minimum(length, steps)
     t_idx = np.arange(steps)[None, :]
-    valid = t_idx < np.minimum(length, steps)[:, None]
+    valid = t_idx < ln[:, None]
     x = trace[..., :N_FEATURES][valid]
     hits = trace[..., N_FEATURES][valid]
     fin = np.broadcast_to(final[:, None], valid.shape)[valid]
     y = (hits >= fin).astype(np.float32)
     if max_rows is not None and x.shape[0] > max_rows:
-        pick = np.sort(np.random.default_rng(seed).choice(x.shape[0], max_rows, replace=False))
+        rng = np.random.default_rng(seed)
+        if long_weight > 0:
+            w = np.broadcast_to((ln / max(1.0, ln.mean()))[:, None] ** long_weight, valid.shape)
+            p = w[valid].astype(np.float64)
+            pick = np.sort(rng.choice(x.shape[0], max_rows, replace=False, p=p / p.sum()))
+        else:
+            pick = np.sort(rng.choice(x.shape[0], max_rows, replace=False))
         x, y = x[pick], y[pick]
     return np.ascontiguousarray(x, dtype=np.float32), np.ascontiguousarray(y, dtype=np.float32)
+
+
+def makespan(lengths: np.ndarray, slots: int) -> float:
+    """Batch time, in expansion times, of the layer-0 launch's schedule modelled as list
+    scheduling: queries in decreasing length (the kernel orders by entry distance, which
+    correlates with length) onto ``slots`` identical wave slots, each query holding its slot for
+    its length."""
+    import heapq
+
+    lengths = np.asarray(lengths, dtype=np.float64)
+    free = [0.0] * max(1, min(int(slots), lengths.size))
+    heapq.heapify(free)
+    end = 0.0
+    for ln in np.sort(lengths)[::-1]:
+        t = heapq.heappop(free) + float(ln)
+        end = max(end, t)
+        heapq.heappush(free, t)
+    return end
 
 
 def oracle_stop(trace: np.ndarray, length: np.ndarray, k: int, target_recall: float):
@@ -91,6 +119,52 @@ def oracle_stop(trace: np.ndarray, length: np.ndarray, k: int, target_recall: fl
         else:
             hi = mid
     return pick(lo)
+
+
+def oracle_stop_tail(trace: np.ndarray, length: np.ndarray, k: int, target_recall: float,
+                     slots: int, plain_length: np.ndarray):
+    """Tail-aware ceiling of any stop rule on this beam: per-query stopping points that keep the
+    mean recall at ``target_recall`` (knowing every query's future) and minimise the batch's
+    makespan (``makespan`` over ``slots`` wave slots) rather than the mean: every cap C on a
+    query's expansions is tried, with the per-expansion price of ``oracle_stop`` bisected under
+    it. Returns a dict with the best makespan, the plain beam's (``plain_length``), their
+    ratio, and the cap and mean expansions that reach it; ``None`` when even the full searches
+    stay below the target."""
+    steps = trace.shape[1]
+    ln = np.minimum(length, steps)
+    hits = trace[..., N_FEATURES].astype(np.float64)
+    t = np.arange(1, steps + 1, dtype=np.float64)[None, :]
+    rows = np.arange(trace.shape[0])
+
+    def pick(lam: float, cap: float):
+        valid = (t <= ln[:, None]) & (t <= cap)
+        score = np.where(valid, hits - lam * t, -np.inf)
+        j = np.argmax(score, axis=1)
+        return t[0, j], hits[rows, j].mean() / k
+
+    plain_span = makespan(np.minimum(plain_length, steps), slots)
+    best = None
+    for cap in sorted(set(int(c) for c in np.linspace(8, int(ln.max()), 40))):
+        stops, rec = pick(0.0, cap)
+        if rec < target_recall:
+            continue
+        lo, hi = 0.0, float(k)
+        for _ in range(40):
+            mid = 0.5 * (lo + hi)
+            if pick(mid, cap)[1] >= target_recall:
+                lo = mid
+            else:
+                hi = mid
+        stops, rec = pick(lo, cap)
+        span = makespan(stops, slots)
+        if best is None or span < best["makespan"]:
+            best = {"makespan": span, "cap": cap, "mean_expansions": float(stops.mean()),
+                    "recall": float(rec)}
+    if best is None:
+        return None
+    best["plain_makespan"] = plain_span
+    best["max_speedup_in_batch_time"] = plain_span / best["makespan"]
+    return best
 
 
 def init_model(x: np.ndarray, seed: int = 0) -> np.ndarray:

@@ -249,6 +249,12 @@ int hm_model_size(void);
  * min_steps layer-0 expansions ran. */
 int hm_set_stop_model(hm_index *index, const float *params, float threshold, int32_t min_steps);
 
+/* Caps the stopped beam at max_steps layer-0 expansions (0: no cap), a second part of the stop
+ * rule (no counterpart in the reference): the launch's time follows its longest queries, and a
+ * cap chosen on training traces bounds them. Applies only while a stop model is installed and
+ * to learned = 1 batches. */
+int hm_set_stop_cap(hm_index *index, int32_t max_steps);
+
 /* Plain register-beam search (unfiltered, k <= ef <= 64, device buffers) that records, for
  * each query and each of its first `steps` layer-0 expansions, the 16 stop features, the number
  * of its current top-k whose distance is <= d_thr[query] (the exact k-th distance plus the

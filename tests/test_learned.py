@@ -64,3 +64,36 @@ def test_oracle_stop_on_hand_traces() -> None:
     exp, rec = oracle_stop(trace, length, 2, 0.75)
     assert rec == 0.75 and exp == 1.5
     assert oracle_stop(trace, np.array([1, 1]), 2, 0.8) is None
+
+
+def test_makespan_of_list_scheduling() -> None:
+    from hnsw_amd.learned import makespan
+
+    rng = np.random.default_rng(3)
+    ln = rng.integers(1, 100, size=500).astype(np.float64)
+    assert makespan(ln, 1) == ln.sum()  # one slot runs everything in turn
+    assert makespan(ln, 1000) == ln.max()  # a slot per query: the longest sets the time
+    m = makespan(ln, 16)
+    # longest-first list scheduling is within 4/3 of the optimum, itself at least the larger of
+    # the mean load and the longest query
+    lower = max(ln.sum() / 16, ln.max())
+    assert lower <= m <= 4.0 / 3.0 * lower + 1e-9
+    assert makespan([5, 5, 4, 4, 3, 3, 3], 3) == 11  # the textbook case: 5+4, 5+4, 3+3+3 -> 11
+
+
+def test_tail_aware_stop_ceiling() -> None:
+    from hnsw_amd.learned import N_FEATURES, oracle_stop_tail
+
+    # 40 queries, hits reach k at step s_q and stay; the full searches run 2 s_q steps.
+    k, steps = 10, 64
+    rng = np.random.default_rng(4)
+    s_q = rng.integers(4, 30, size=40)
+    trace = np.zeros((40, steps, N_FEATURES + 2), dtype=np.float32)
+    for i, s in enumerate(s_q):
+        trace[i, :, N_FEATURES] = np.minimum(np.arange(1, steps + 1) * k // s, k)
+    length = 2 * s_q
+    out = oracle_stop_tail(trace, length, k, 1.0, 4, length)
+    assert out is not None and out["recall"] >= 1.0
+    # stopping at s_q is free: the batch time can halve at most
+    assert 1.0 < out["max_speedup_in_batch_time"] <= 2.0 + 1e-9
+    assert oracle_stop_tail(trace, np.minimum(length, 3), k, 1.0, 4, length) is None
