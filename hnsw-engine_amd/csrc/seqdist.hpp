@@ -1,10 +1,11 @@
 // The reference's float64 distance (numba_engine.py:57-70) of one row per lane: float64, left
 // to right, no fused multiply-add: l2 = sum (x_i - q_i)^2, ip = -sum x_i q_i, cosine =
-// 1 - sum x_i q_i. The row is read in 16-byte chunks, four at a time, and the next four are in
-// flight while the current ones are summed, so a row costs about one memory latency instead of
-// one per chunk. The query (dim float64 or float32 values in global memory, the same for every
-// lane) is read through the scalar cache into scalar registers, off the chain of dependent
-// adds; a float32 query value converts to float64 exactly, as the reference's does.
+// 1 - sum x_i q_i. The row is read in 16-byte chunks, eight at a time, and the next eight are
+// in flight while the current ones are summed, so a row of up to 128 bytes costs one memory
+// latency, and a longer one about one per 128 bytes rather than one per chunk. The query (dim
+// float64 or float32 values in global memory, the same for every lane) is read through the
+// scalar cache into scalar registers, off the chain of dependent adds; a float32 query value
+// converts to float64 exactly, as the reference's does.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -43,6 +44,29 @@ HM_DEV void seq_chunk(double &acc, uint4 w, const QT *q, int i0, int dim, bool l
     }
 }
 
+// seq_chunk for a chunk wholly inside the row: no per-element bounds checks, so the chunk's
+// query values load together (one wait) instead of one scalar round trip per element.
+template <bool U8, class QT>
+HM_DEV void seq_chunk_full(double &acc, uint4 w, const QT *q, int i0, bool l2) {
+#pragma clang fp contract(off)
+    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+    constexpr int kPer = U8 ? 16 : 4;
+    double qv[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) qv[e] = qval(q, i0 + e);
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const double x = U8 ? (double)((ws[e >> 2] >> (8 * (e & 3))) & 255u)
+                            : (double)__uint_as_float(ws[e]);
+        if (l2) {
+            const double d = x - qv[e];
+            acc = acc + d * d;
+        } else {
+            acc = acc + x * qv[e];
+        }
+    }
+}
+
 template <bool U8, class QT>
 HM_DEV double seq_row(const void *row, const QT *q, int dim, int metric) {
 #pragma clang fp contract(off)
@@ -61,7 +85,9 @@ HM_DEV double seq_row(const void *row, const QT *q, int dim, int metric) {
         }
     } else {
         constexpr int kPer = U8 ? 16 : 4;
-        constexpr int kB = 4;
+        // Chunks in flight: a whole 8-bit row up to 128 values (one memory round trip); float
+        // rows stream eight chunks ahead.
+        constexpr int kB = 8;
         const uint4 *r4 = (const uint4 *)row;
         const int nch = (dim + kPer - 1) / kPer;
         uint4 cur[kB];
@@ -74,8 +100,11 @@ HM_DEV double seq_row(const void *row, const QT *q, int dim, int metric) {
             for (int t = 0; t < kB; ++t)
                 if (b0 + kB + t < nch) nxt[t] = r4[b0 + kB + t];
 #pragma unroll
-            for (int t = 0; t < kB; ++t)
-                if (b0 + t < nch) seq_chunk<U8, QT>(acc, cur[t], q, (b0 + t) * kPer, dim, l2);
+            for (int t = 0; t < kB; ++t) {
+                const int i0 = (b0 + t) * kPer;
+                if (i0 + kPer <= dim) seq_chunk_full<U8, QT>(acc, cur[t], q, i0, l2);
+                else if (i0 < dim) seq_chunk<U8, QT>(acc, cur[t], q, i0, dim, l2);
+            }
 #pragma unroll
             for (int t = 0; t < kB; ++t) cur[t] = nxt[t];
         }
