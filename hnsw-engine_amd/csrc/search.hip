@@ -2205,20 +2205,37 @@ __global__ void reset_queue_kernel(int32_t *q) { q[0] = 0; }
 // Orders queries by their layer-0 entry key, largest first (a counting sort on 256 buckets of
 // the key's log2 with 3 fraction bits; order within a bucket is arbitrary): the farther a query
 // enters layer 0 from its nearest neighbours, the more expansions it tends to take, and taking
-// long queries first shortens a launch's tail. One workgroup of 1024 threads; it also resets the
-// work queue of the layer-0 launch that follows.
+// long queries first shortens a launch's tail. With `relative` (inner product, whose keys are
+// 2^31 minus the dot product and so share one log bucket) the key's excess over the batch's
+// smallest key is bucketed instead. One workgroup of 1024 threads; it also resets the work
+// queue of the layer-0 launch that follows.
 __global__ __launch_bounds__(1024) void order_by_entry_kernel(const uint64_t *key, int nq,
-                                                              int32_t *order, int32_t *queue) {
+                                                              int relative, int32_t *order,
+                                                              int32_t *queue) {
     constexpr int kKeep = 16384;  // batches up to this size keep each query's bucket in LDS
     __shared__ int32_t count[256];
     __shared__ int32_t start[256];
     __shared__ uint8_t kept[kKeep];
+    __shared__ unsigned long long kmin;
     const bool keep = nq <= kKeep;
     const int t = threadIdx.x;
-    if (t == 0) queue[0] = 0;
+    if (t == 0) {
+        queue[0] = 0;
+        kmin = 0ull;
+    }
     if (t < 256) count[t] = 0;
     __syncthreads();
-    auto bucket = [](uint64_t k) {
+    if (relative) {
+        if (t == 0) kmin = ~0ull;
+        __syncthreads();
+        unsigned long long m = ~0ull;
+        for (int i = t; i < nq; i += 1024) m = min(m, (unsigned long long)key[i]);
+        atomicMin(&kmin, m);
+        __syncthreads();
+    }
+    const uint64_t base = kmin;
+    auto bucket = [base](uint64_t k0) {
+        const uint64_t k = k0 - base;
         const uint32_t v = (uint32_t)min<uint64_t>(k, 0xFFFFFFFEull) + 1u;  // >= 1
         const int e = 31 - __clz(v);
         const int frac = e >= 3 ? (int)((v >> (e - 3)) & 7u) : (int)((v << (3 - e)) & 7u);
@@ -2508,7 +2525,7 @@ hipError_t search_launch(int mode, bool filt, const SearchArgs &a, int blocks, h
     hipLaunchKernelGGL(descend_kernel, dim3((a.nq + kDescentGroups - 1) / kDescentGroups),
                        dim3(256), 0, s, a);
     hipLaunchKernelGGL(order_by_entry_kernel, dim3(1), dim3(1024), 0, s, a.pre_key, a.nq,
-                       const_cast<int32_t *>(a.order), a.queue);
+                       a.metric == kIP ? 1 : 0, const_cast<int32_t *>(a.order), a.queue);
     SearchArgs l = a;
     l.phase = 2;
     hipLaunchKernelGGL(fn, dim3(blocks), dim3(64), a.lds_per_wave, s, l);

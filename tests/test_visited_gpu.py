@@ -171,6 +171,25 @@ def test_two_phase_inner_product_matches_oracle(m16d128_ip, ef: int) -> None:
     _same(eng.search_many(queries, 10, ef), want, f"ip ef={ef}")
 
 
+@pytest.mark.parametrize("which,metric", [("m16d128", "l2"), ("m16d128_ip", "ip")])
+def test_two_phase_batch_beyond_the_order_kernels_lds(request, which: str, metric: str) -> None:
+    """17,000 queries: the order kernel recomputes each query's bucket instead of keeping it in
+    LDS (more than 16,384 queries), and inner-product keys are bucketed relative to the batch's
+    smallest key."""
+    eng, data, _ = request.getfixturevalue(which)
+    rng = np.random.default_rng(53)
+    rows = data[rng.integers(0, data.shape[0], size=17_000)]
+    queries = np.clip(rows + rng.integers(-20, 21, size=rows.shape), 0, 255).astype(np.float32)
+    want = _oracle(eng, queries, 10, 32, metric)
+    print(f"case {which} two-phase 17000 queries", flush=True)
+    try:
+        _tune(eng, waves_per_cu=1)
+        got = eng.search_many(queries, 10, 32)
+    finally:
+        _tune(eng)
+    _same(got, want, f"{which} 17000 queries")
+
+
 @pytest.mark.parametrize("ef", [48, 100])
 def test_padded_rows_match_oracle(m16d120, ef: int) -> None:
     """120 dimensions on the constant shape, in one launch and after the batched descent."""
