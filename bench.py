@@ -175,8 +175,8 @@ def main() -> int:
         # Expansion caps: the tail-aware oracle's cap on the training traces (largest ef), and
         # looser ones; 0 is no cap.
         slots = 16 * torch.cuda.get_device_properties(0).multi_processor_count
-        t_train = lrn.oracle_stop_tail(trace, length, k, args.target_recall, slots, length)
-        c0 = int(t_train["cap"]) if t_train else 0
+        cap_plan = lrn.oracle_stop_tail(trace, length, k, args.target_recall, slots, length)
+        c0 = int(cap_plan["cap"]) if cap_plan else 0
         caps = sorted({0, c0, int(1.25 * c0), int(1.5 * c0)} if c0 else {0})
         log(f"[bench] stop caps from the training traces: {caps}")
         lsweep = []

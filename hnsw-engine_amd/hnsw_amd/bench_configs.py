@@ -1,5 +1,6 @@
 """Measurements of the secondary workloads (the headline one is ``bench.py`` at the repository
-root). ``python -m hnsw_amd.bench_configs filtered|quantized|mixed`` prints one JSON line.
+root). ``python -m hnsw_amd.bench_configs filtered|quantized|mixed|codes [ef ...]`` prints one
+JSON line.
 
 ``filtered``: 100,000 x 784 8-bit vectors (about half exact zeros, 10 prototypes plus uniform
 noise), 5,000 queries, k = 10, ef = 128, L2, M = 16, ef_construction = 200; masks of
@@ -141,6 +142,47 @@ def quantized(log=print) -> dict:
             "rows": rows}
 
 
+HBM_PEAK_GBS = 8000.0
+
+
+def _hbm(bytes_per_query: float, qps: float) -> dict:
+    """Algorithmic bytes per query (rows gathered for each distance, one layer-0 links row per
+    expansion) at the measured rate, against the 8 TB/s HBM peak."""
+    gbs = bytes_per_query * qps / 1e9
+    return {"bytes_per_query": bytes_per_query, "achieved_gbs": gbs,
+            "frac": gbs / HBM_PEAK_GBS}
+
+
+def codes(log=print, efs=(128,)) -> dict:
+    """The codes traversal alone (``batch_search_codes``: uint8 codes, float64 code-space
+    queries, the reference's ``numba_engine.py:674-701``) on angular_400k at each ef, k = ef as
+    the quantized search asks for its re-rank candidates; host wall time of the whole batch,
+    best of 3."""
+    from .quantization import ScalarQuantizer
+    from .workloads import WORKLOADS, load_or_build
+
+    w = WORKLOADS["angular_400k"]
+    eng, data, queries = load_or_build(w, log=log)
+    quant = ScalarQuantizer.fit(data)
+    cb = quant.encode(data)
+    qcode = quant.transform_queries(queries, w.metric)
+    nq = queries.shape[0]
+    rows = []
+    for ef in efs:
+        d0, h0 = eng.distance_computations, eng.hops
+        t, _ = _best(lambda: eng.batch_search_codes(cb, qcode, ef, ef))
+        runs = 4  # _best runs the batch once untimed and three times timed
+        dq = (eng.distance_computations - d0) / runs / nq
+        hq = (eng.hops - h0) / runs / nq
+        qps = nq / t
+        row = {"ef": ef, "qps": qps, "seconds": t, "mean_distances": dq, "mean_hops": hq,
+               "roofline": _hbm(dq * w.dim + hq * 4 * 2 * w.M, qps)}
+        rows.append(row)
+        log(f"[codes] {json.dumps(row)}")
+    return {"workload": f"{w.name}: {w.n} x {w.dim} uint8 codes, {nq} queries, k = ef",
+            "rows": rows}
+
+
 def mixed_batch(w, nq: int = 100_000):
     """The mixed batch on workload ``w`` (fixed seeds): ``(queries, k, ef, pool, mask_index)``
     with ef drawn from {16, 32, 64, 128, 256, 512} (raised to k), k from {1, 10, 100}, and half
@@ -170,20 +212,22 @@ def mixed(log=print) -> dict:
     nq = queries.shape[0]
     t, got = _best(lambda: eng.search_many(queries, k, ef, mask_pool=pool, mask_index=midx), 2)
     plain = midx < 0
+    dq, hq = float(got[2].mean()), float(got[3].mean())
     return {"workload": f"{w.name}: {nq} mixed queries (ef 16-512, k 1/10/100, "
                         f"{int((~plain).sum())} filtered at 5-50 %)",
-            "qps": nq / t, "seconds": t,
-            "mean_distances": float(got[2].mean()), "mean_hops": float(got[3].mean())}
+            "qps": nq / t, "seconds": t, "mean_distances": dq, "mean_hops": hq,
+            "roofline": _hbm(dq * w.dim + hq * 4 * 2 * w.M, nq / t)}
 
 
 def main(argv: list[str]) -> int:
     which = argv[1] if len(argv) > 1 else "filtered"
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-    runs = {"filtered": filtered, "quantized": quantized, "mixed": mixed}
+    runs = {"filtered": filtered, "quantized": quantized, "mixed": mixed, "codes": codes}
     if which not in runs:
         print(f"unknown workload {which!r}; one of {sorted(runs)}", file=sys.stderr)
         return 2
-    print(json.dumps(runs[which](log=log)), flush=True)
+    extra = {"efs": tuple(int(v) for v in argv[2:])} if which == "codes" and argv[2:] else {}
+    print(json.dumps(runs[which](log=log, **extra)), flush=True)
     return 0
 
 
