@@ -42,6 +42,7 @@ struct hm_index {
     hm::DevBuf<float> q32;
     hm::DevBuf<double> q64;
     hm::DevBuf<int32_t> kq, efq, midx, dc, hops, fail, subset;
+    hm::DevBuf<uint8_t> boost;  // per position of a mixed batch: order boosts
     hm::DevBuf<uint32_t> maskbits;
     hm::DevBuf<int64_t> ids, cand, rids;
     hm::DevBuf<double> dists, rdists;

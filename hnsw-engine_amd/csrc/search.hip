@@ -2210,8 +2210,9 @@ __global__ void reset_queue_kernel(int32_t *q) { q[0] = 0; }
 // smallest key is bucketed instead. One workgroup of 1024 threads; it also resets the work
 // queue of the layer-0 launch that follows.
 __global__ __launch_bounds__(1024) void order_by_entry_kernel(const uint64_t *key, int nq,
-                                                              int relative, int32_t *order,
-                                                              int32_t *queue) {
+                                                              int relative,
+                                                              const uint8_t *boost,
+                                                              int32_t *order, int32_t *queue) {
     constexpr int kKeep = 16384;  // batches up to this size keep each query's bucket in LDS
     __shared__ int32_t count[256];
     __shared__ int32_t start[256];
@@ -2242,7 +2243,7 @@ __global__ __launch_bounds__(1024) void order_by_entry_kernel(const uint64_t *ke
         return 255 - min(255, e * 8 + frac);  // descending key -> ascending bucket
     };
     for (int i = t; i < nq; i += 1024) {
-        const int b = bucket(key[i]);
+        const int b = boost ? max(0, bucket(key[i]) - (int)boost[i]) : bucket(key[i]);
         if (keep) kept[i] = (uint8_t)b;
         atomicAdd(&count[b], 1);
     }
@@ -2263,7 +2264,8 @@ __global__ __launch_bounds__(1024) void order_by_entry_kernel(const uint64_t *ke
     }
     __syncthreads();
     for (int i = t; i < nq; i += 1024) {
-        const int b = keep ? (int)kept[i] : bucket(key[i]);
+        const int b = keep ? (int)kept[i]
+                           : (boost ? max(0, bucket(key[i]) - (int)boost[i]) : bucket(key[i]));
         order[atomicAdd(&start[b], 1)] = i;
     }
 }
@@ -2525,7 +2527,8 @@ hipError_t search_launch(int mode, bool filt, const SearchArgs &a, int blocks, h
     hipLaunchKernelGGL(descend_kernel, dim3((a.nq + kDescentGroups - 1) / kDescentGroups),
                        dim3(256), 0, s, a);
     hipLaunchKernelGGL(order_by_entry_kernel, dim3(1), dim3(1024), 0, s, a.pre_key, a.nq,
-                       a.metric == kIP ? 1 : 0, const_cast<int32_t *>(a.order), a.queue);
+                       a.metric == kIP ? 1 : 0, a.order_boost, const_cast<int32_t *>(a.order),
+                       a.queue);
     SearchArgs l = a;
     l.phase = 2;
     hipLaunchKernelGGL(fn, dim3(blocks), dim3(64), a.lds_per_wave, s, l);

@@ -54,6 +54,9 @@ struct SearchArgs {
     // `order` (longest expected first). phase 0: one launch does both.
     int phase;
     const int32_t *order;  // queue position -> query index (null: identity)
+    // Per position: log-buckets by which the order moves the query earlier (null: none); a
+    // filtered query's work grows as its mask's selectivity falls.
+    const uint8_t *order_boost;
     uint64_t *pre_key;
     uint32_t *pre_id;
     int32_t *pre_dc, *pre_hops;
