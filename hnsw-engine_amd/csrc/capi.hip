@@ -12,6 +12,7 @@
 #include "../../include/hnsw_mi355x.h"
 #include "common.hpp"
 #include "exact_tile.hpp"
+#include "maskbits.hpp"
 #include "index.hpp"
 #include "search.hpp"
 
@@ -355,14 +356,6 @@ hipError_t order_after(hm_index *h, hipStream_t s) {
 }
 
 // Packs boolean rows (n bytes each) into uint32 bitsets.
-std::vector<uint32_t> pack_masks(const uint8_t *pool, int64_t rows, int64_t n) {
-    const int64_t words = (n + 31) / 32;
-    std::vector<uint32_t> out((size_t)(rows * words), 0u);
-    for (int64_t r = 0; r < rows; ++r)
-        for (int64_t i = 0; i < n; ++i)
-            if (pool[r * n + i]) out[(size_t)(r * words + i / 32)] |= 1u << (i % 32);
-    return out;
-}
 
 int check_opts(const hm_index *h, const hm_search_opts *o) {
     if (!o || o->nq < 0) return invalid("bad search options");

@@ -34,6 +34,7 @@
 #include "exact_tile.hpp"
 #include "index.hpp"
 #include "keys.hpp"
+#include "maskbits.hpp"
 #include "wave.hpp"
 
 typedef __attribute__((ext_vector_type(4))) int i32x4;
@@ -1083,9 +1084,7 @@ extern "C" int hm_exact(const float *data, int64_t n, int32_t dim, int32_t metri
     HM_TRY(hipMemcpy(dx.p, data, (size_t)n * dim * 4, hipMemcpyHostToDevice), "copy data");
     HM_TRY(hipMemcpy(dq.p, queries, (size_t)nq * dim * 4, hipMemcpyHostToDevice), "copy queries");
     if (allowed) {
-        std::vector<uint32_t> b((size_t)(n + 31) / 32, 0u);
-        for (int64_t i = 0; i < n; ++i)
-            if (allowed[i]) b[i / 32] |= 1u << (i % 32);
+        const std::vector<uint32_t> b = pack_masks(allowed, 1, n);
         HM_TRY(bits.alloc(b.size()), "alloc");
         HM_TRY(hipMemcpy(bits.p, b.data(), b.size() * 4, hipMemcpyHostToDevice), "copy mask");
     }
@@ -1149,10 +1148,10 @@ extern "C" int hm_index_exact(hm_index *h, const float *queries, int64_t nq, int
     HM_TRY(od.alloc((size_t)nq * k), "alloc");
     HM_TRY(hipMemcpyAsync(dq.p, queries, (size_t)nq * h->dim * 4, hipMemcpyHostToDevice, h->stream),
            "copy queries");
+    // The packed mask outlives the search: its asynchronous copy may still read it.
+    std::vector<uint32_t> b;
     if (allowed) {
-        std::vector<uint32_t> b((size_t)(h->n + 31) / 32, 0u);
-        for (int64_t i = 0; i < h->n; ++i)
-            if (allowed[i]) b[i / 32] |= 1u << (i % 32);
+        b = pack_masks(allowed, 1, h->n);
         HM_TRY(bits.alloc(b.size()), "alloc");
         HM_TRY(hipMemcpyAsync(bits.p, b.data(), b.size() * 4, hipMemcpyHostToDevice, h->stream),
                "copy mask");

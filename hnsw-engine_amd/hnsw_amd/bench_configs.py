@@ -53,8 +53,10 @@ def filtered(log=print) -> dict:
     eng, data, queries = load_or_build(w, log=log)
     n, k, ef = data.shape[0], 10, 128
     planner = FilterPlanner()
-    calib = planner.calibrate(eng, queries[:1000], k, ef,
-                              selectivities=(0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.5))
+    # Calibrated on the batch it plans for: both paths' costs depend on the batch size (the
+    # in-graph search needs thousands of queries to fill the GPU's wave slots).
+    calib = planner.calibrate(eng, queries, k, ef,
+                              selectivities=(0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.3, 0.5))
     log(f"[filtered] planner thresholds: random {planner.random_threshold}, "
         f"correlated {planner.correlated_threshold}; timings {calib}")
     rows = []

@@ -78,10 +78,12 @@ class FilterPlanner:
 
     def calibrate(self, engine, queries: np.ndarray, k: int, ef: int,
                   selectivities=(0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.5), repeats: int = 2):
-        """Times both paths on random masks of each selectivity (queries: a sample of the
-        workload) and sets ``random_threshold`` to the smallest selectivity where the in-graph
-        search is faster, checking larger ones; the correlated threshold keeps its ratio to it.
-        Returns the timings ``[(selectivity, graph_s, exact_s)]``."""
+        """Times both paths on random masks of each selectivity (queries: a batch of the size
+        the planner will serve, since both costs depend on it: the in-graph search needs
+        thousands of queries to fill the GPU) and sets ``random_threshold`` to the smallest
+        selectivity where the in-graph search is faster, checking larger ones; the correlated
+        threshold keeps its ratio to it. Returns the timings ``[(selectivity, graph_s,
+        exact_s)]``."""
         n = engine.size
         rows = []
         for i, s in enumerate(sorted(selectivities)):
