@@ -61,13 +61,13 @@ def main() -> int:
     ap.add_argument("--workload", default="clustered_1m")
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--target-recall", type=float, default=0.95)
-    ap.add_argument("--sweep", default="16,24,32,40,48,56,64,96,128,256,512")
+    ap.add_argument("--sweep", default="16,24,32,40,42,44,46,48,56,64,96,128,256,512")
     ap.add_argument("--cpu-seconds", type=float, default=8.0,
                     help="target CPU time of each CPU-baseline leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--train-queries", type=int, default=10_000)
-    ap.add_argument("--learned-ef", default="56,64")
+    ap.add_argument("--learned-ef", default="48,56,64")
     ap.add_argument("--thresholds", default="0.5,0.6,0.7,0.8,0.85,0.9,0.95")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB on the device three times (run under rocprofv3 --pmc "
