@@ -1,4 +1,5 @@
 // C ABI: index upload (the reference's attach_native) and the batch search entry points.
+#include <emmintrin.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -47,6 +48,7 @@ namespace {
 struct Batch {
     const float *q32 = nullptr;
     const double *q64 = nullptr;
+    const uint8_t *q8 = nullptr;  // exact mode: query bytes in place of q32
     const int32_t *k = nullptr, *ef = nullptr, *midx = nullptr;
     const uint32_t *mask_bits = nullptr;
     int64_t mask_words = 0;
@@ -189,6 +191,7 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap,
     a.dimp = h->dimp;
     a.metric = h->metric;
     a.q_f32 = b.q32;
+    a.q_u8 = b.q8;
     a.q_f64 = b.q64;
     a.nq = (int)b.nq;
     a.ef_q = b.ef;
@@ -371,7 +374,8 @@ int check_opts(const hm_index *h, const hm_search_opts *o) {
 // exact re-rank scores them there; ids / dists then receive its best rerank_k per query.
 int search_host(hm_index *h, int mode, const float *q32, const double *q64,
                 const hm_search_opts *o, int64_t *ids, double *dists, int64_t *dcount,
-                int64_t *hops, const float *rerank_q = nullptr, int rerank_k = 0) {
+                int64_t *hops, const float *rerank_q = nullptr, int rerank_k = 0,
+                const uint8_t *q8 = nullptr) {
     int st = check_opts(h, o);
     if (st) return st;
     const int64_t nq = o->nq;
@@ -433,7 +437,11 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
         }
     }
     const size_t qn = (size_t)nq * h->dim;
-    if (q32) {
+    if (q8) {  // exact mode: one byte per value (a quarter of the float32 upload)
+        HM_TRY(h->q8.ensure(qn), "alloc");
+        HM_TRY(hipMemcpyAsync(h->q8.p, q8, qn, hipMemcpyHostToDevice, s), "copy queries");
+        b.q8 = h->q8.p;
+    } else if (q32) {
         HM_TRY(h->q32.ensure(qn), "alloc");
         HM_TRY(hipMemcpyAsync(h->q32.p, q32, qn * 4, hipMemcpyHostToDevice, s), "copy queries");
         b.q32 = h->q32.p;
@@ -538,8 +546,36 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
     st = run(nullptr, std::min<int64_t>(h->n, kDefaultSpill), 0, 2);
     if (st) return st;
     std::vector<int32_t> failv(nq), dcv(nq), hv(nq);
-    HM_TRY(hipMemcpyAsync(failv.data(), h->fail.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
-    HM_TRY(hipStreamSynchronize(s), "search");
+    // The failure flags and everything the caller gets back are queued behind the search and
+    // fetched with one synchronisation; a batch with failed queries (rare) runs them again and
+    // fetches once more.
+    auto fetch = [&]() -> int {
+        HM_TRY(hipMemcpyAsync(failv.data(), h->fail.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
+        if (rerank_q) {
+            const size_t rn = (size_t)nq * rerank_k;
+            HM_TRY(h->q32.ensure((size_t)nq * h->dim), "alloc");
+            HM_TRY(h->rids.ensure(rn), "alloc");
+            HM_TRY(h->rdists.ensure(rn), "alloc");
+            HM_TRY(hipMemcpyAsync(h->q32.p, rerank_q, (size_t)nq * h->dim * 4,
+                                  hipMemcpyHostToDevice, s),
+                   "copy queries");
+            HM_TRY(rerank_device(h, h->q32.p, h->ids.p, nq, o->kstride, rerank_k, h->rids.p,
+                                 h->rdists.p, s),
+                   "rerank launch");
+            HM_TRY(hipMemcpyAsync(ids, h->rids.p, rn * 8, hipMemcpyDeviceToHost, s), "copy ids");
+            HM_TRY(hipMemcpyAsync(dists, h->rdists.p, rn * 8, hipMemcpyDeviceToHost, s), "copy");
+        } else {
+            HM_TRY(hipMemcpyAsync(ids, h->ids.p, on * 8, hipMemcpyDeviceToHost, s), "copy ids");
+            HM_TRY(hipMemcpyAsync(dists, h->dists.p, on * 8, hipMemcpyDeviceToHost, s),
+                   "copy dists");
+        }
+        HM_TRY(hipMemcpyAsync(dcv.data(), h->dc.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
+        HM_TRY(hipMemcpyAsync(hv.data(), h->hops.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
+        HM_TRY(hipStreamSynchronize(s), "search");
+        return HM_OK;
+    };
+    st = fetch();
+    if (st) return st;
     // Queries that failed (candidates beyond the spill, or evicted ties beyond the tie list of
     // the kFast register beams) run again, as one batch per class, on the general register beam
     // or the general kernel with room for every node (no query can hold more candidates than
@@ -553,37 +589,44 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
         const int slots = (int)std::max<int64_t>(1, std::min<int64_t>(1 << 20, budget / (h->n * 12)));
         st = run(&redo, h->n, slots, 1);
         if (st) return st;
-        HM_TRY(hipMemcpyAsync(failv.data(), h->fail.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
-        HM_TRY(hipStreamSynchronize(s), "search");
+        st = fetch();
+        if (st) return st;
         for (int64_t i = 0; i < nq; ++i)
             if (failv[i]) return invalid("candidate spill overflow");
     }
-    if (rerank_q) {
-        const size_t rn = (size_t)nq * rerank_k;
-        HM_TRY(h->q32.ensure((size_t)nq * h->dim), "alloc");
-        HM_TRY(h->rids.ensure(rn), "alloc");
-        HM_TRY(h->rdists.ensure(rn), "alloc");
-        HM_TRY(hipMemcpyAsync(h->q32.p, rerank_q, (size_t)nq * h->dim * 4, hipMemcpyHostToDevice,
-                              s),
-               "copy queries");
-        HM_TRY(rerank_device(h, h->q32.p, h->ids.p, nq, o->kstride, rerank_k, h->rids.p,
-                             h->rdists.p, s),
-               "rerank launch");
-        HM_TRY(hipMemcpyAsync(ids, h->rids.p, rn * 8, hipMemcpyDeviceToHost, s), "copy ids");
-        HM_TRY(hipMemcpyAsync(dists, h->rdists.p, rn * 8, hipMemcpyDeviceToHost, s), "copy");
-    } else {
-        HM_TRY(hipMemcpyAsync(ids, h->ids.p, on * 8, hipMemcpyDeviceToHost, s), "copy ids");
-        HM_TRY(hipMemcpyAsync(dists, h->dists.p, on * 8, hipMemcpyDeviceToHost, s), "copy dists");
-    }
-    HM_TRY(hipMemcpyAsync(dcv.data(), h->dc.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
-    HM_TRY(hipMemcpyAsync(hv.data(), h->hops.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
     HM_TRY(order_after(h, s), "stream order");
-    HM_TRY(hipStreamSynchronize(s), "search");
     for (int64_t i = 0; i < nq; ++i) {
         if (dcount) dcount[i] = dcv[i];
         if (hops) hops[i] = hv[i];
     }
     return HM_OK;
+}
+
+// Converts float32 values to bytes when every one is an integer in 0..255 (NaN and -0.0 as
+// all_u8_integers treats them), four at a time by 128-bit compares and packs; false otherwise.
+bool pack_u8_integers(const float *v, size_t count, uint8_t *out) {
+    const __m128 lo = _mm_setzero_ps(), hi = _mm_set1_ps(255.0f);
+    __m128 bad = _mm_setzero_ps();
+    size_t i = 0;
+    for (; i + 16 <= count; i += 16) {
+        __m128i w[4];
+        for (int t = 0; t < 4; ++t) {
+            const __m128 x = _mm_loadu_ps(v + i + 4 * t);
+            const __m128 c = _mm_min_ps(_mm_max_ps(x, lo), hi);  // NaN -> 0
+            w[t] = _mm_cvttps_epi32(c);
+            bad = _mm_or_ps(bad, _mm_cmpneq_ps(_mm_cvtepi32_ps(w[t]), x));
+        }
+        const __m128i p = _mm_packus_epi16(_mm_packs_epi32(w[0], w[1]), _mm_packs_epi32(w[2], w[3]));
+        _mm_storeu_si128((__m128i *)(out + i), p);
+    }
+    if (_mm_movemask_ps(bad)) return false;
+    for (; i < count; ++i) {
+        const float x = v[i];
+        const float c = std::min(std::max(x, 0.0f), 255.0f);
+        if (c != x || (float)(int)c != c) return false;
+        out[i] = (uint8_t)(int)c;
+    }
+    return true;
 }
 
 // Branch-free over blocks so that the compiler vectorises it (NaN clamps to 0 and differs).
@@ -737,9 +780,18 @@ int hm_queries_exact(const float *q, int64_t count) {
 int hm_search(hm_index *h, const float *queries, const hm_search_opts *o, int64_t *ids,
               double *dists, int64_t *dcount, int64_t *hops) {
     if (!h || !queries || !o || !ids || !dists) return invalid("null argument");
-    const bool exact = h->exact_u8 && hm_queries_exact(queries, o->nq * h->dim);
-    return search_host(h, exact ? kExactU8 : kSeqF32, queries, nullptr, o, ids, dists, dcount,
-                       hops);
+    if (h->exact_u8 && o->nq > 0) {
+        // Integer queries 0..255 travel as bytes: checked and packed in one pass into pinned
+        // memory, a quarter of the float32 bytes over the link. (Every host-buffer call returns
+        // synchronised, so no copy from the staging buffer is still in flight.)
+        const size_t count = (size_t)o->nq * h->dim;
+        HM_TRY(hipSetDevice(h->device), "hipSetDevice");
+        HM_TRY(h->pin_q8.ensure(count), "pinned allocation");
+        if (pack_u8_integers(queries, count, h->pin_q8.p))
+            return search_host(h, kExactU8, nullptr, nullptr, o, ids, dists, dcount, hops,
+                               nullptr, 0, h->pin_q8.p);
+    }
+    return search_host(h, kSeqF32, queries, nullptr, o, ids, dists, dcount, hops);
 }
 
 namespace {

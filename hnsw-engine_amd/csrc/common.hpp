@@ -42,6 +42,29 @@ struct DevBuf {
     }
 };
 
+// Grow-only pinned host buffer (staging for host-to-device copies at full link speed).
+template <class T>
+struct PinBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t count) {
+        if (count <= cap && p) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = count ? count : 1;
+        hipError_t e = hipHostMalloc((void **)&p, want * sizeof(T), hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        cap = want;
+        return hipSuccess;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
 inline int ceil_log2(int64_t v) {
     int r = 0;
     while ((int64_t(1) << r) < v) ++r;

@@ -43,6 +43,9 @@ struct hm_index {
     hm::DevBuf<double> q64;
     hm::DevBuf<int32_t> kq, efq, midx, dc, hops, fail, subset;
     hm::DevBuf<uint8_t> boost;  // per position of a mixed batch: order boosts
+    // 8-bit queries of hm_search (exact mode): packed on the host into pinned memory, then copied
+    hm::PinBuf<uint8_t> pin_q8;
+    hm::DevBuf<uint8_t> q8;
     hm::DevBuf<uint32_t> maskbits;
     hm::DevBuf<int64_t> ids, cand, rids;
     hm::DevBuf<double> dists, rdists;
@@ -81,6 +84,9 @@ struct hm_index {
         spill_id.release();
         q32.release();
         q64.release();
+        q8.release();
+        pin_q8.release();
+        boost.release();
         maskbits.release();
         ids.release();
         cand.release();

@@ -2029,9 +2029,10 @@ HM_DEV void run_query(const SearchArgs &a, const Lds &L, Visited &V, int qi, int
     // Query into LDS.
     if (MODE == kExactU8) {
         const float *q = a.q_f32 + (size_t)qi * a.dim;
+        const uint8_t *q8 = a.q_u8 + (size_t)qi * a.dim;
         int32_t s = 0;
         for (int i = lane; i < a.dimp; i += 64) {
-            const int v = i < a.dim ? (int)q[i] : 0;
+            const int v = i < a.dim ? (a.q_u8 ? (int)q8[i] : (int)q[i]) : 0;
             L.qb[i] = (uint8_t)v;
             s += v * v;
         }
@@ -2299,10 +2300,11 @@ __global__ __launch_bounds__(256) void descend_kernel(SearchArgs a) {
     uint32_t qw[2] = {0u, 0u};
     {
         const float *q = a.q_f32 + (size_t)qc * a.dim;
+        const uint8_t *q8 = a.q_u8 + (size_t)qc * a.dim;
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             const int i = 8 * j + t;
-            const int v = i < a.dim ? (int)q[i] : 0;
+            const int v = i < a.dim ? (a.q_u8 ? (int)q8[i] : (int)q[i]) : 0;
             qw[t >> 2] |= (uint32_t)v << (8 * (t & 3));
             qq += v * v;
         }

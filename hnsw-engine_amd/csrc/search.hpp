@@ -32,6 +32,7 @@ struct SearchArgs {
     // queries: float32 (kExactU8, kSeqF32) or float64 (kSeqU8), (nq, dim)
     const float *q_f32;
     const double *q_f64;
+    const uint8_t *q_u8;  // exact mode: query bytes (dim per query) in place of q_f32, or null
     int nq;
     const int32_t *ef_q;  // per query, already max(ef, k); null: ef_all
     const int32_t *k_q;   // per query; null: k_all
