@@ -492,35 +492,45 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
     };
     // Runs the queries `which` (all of the batch when null), class by class; spill: candidate
     // spill entries per wave slot, max_slots: a cap on wave slots (0: none).
+    // Host arrays the class launches copy from: they live until the call's final
+    // synchronisation (a re-run reuses them only after the first run's results were fetched).
+    std::vector<int32_t> order;
+    std::vector<uint8_t> boost;
     auto run = [&](const std::vector<int32_t> *which, int64_t spill, int max_slots,
                    int tier) -> int {
         const int64_t count = which ? (int64_t)which->size() : nq;
-        std::vector<int32_t> order;
-        order.reserve(count);
+        // Each query's class once, then a counting sort by class (stable: batch order within
+        // a class).
+        std::vector<uint8_t> cls((size_t)count);
         std::vector<int64_t> start(11, 0);
+        for (int64_t t = 0; t < count; ++t) {
+            const int c = class_of(which ? (*which)[t] : t);
+            cls[(size_t)t] = (uint8_t)c;
+            start[c + 1] += 1;
+        }
         int present = 0;
         for (int c = 0; c < 10; ++c) {
-            start[c] = (int64_t)order.size();
-            for (int64_t t = 0; t < count; ++t) {
-                const int64_t i = which ? (*which)[t] : t;
-                if (class_of(i) == c) order.push_back((int32_t)i);
-            }
-            if (c >= 5)  // filtered: most selective masks first (the queue order of one launch)
-                std::stable_sort(order.begin() + start[c], order.end(),
-                                 [&](int32_t x, int32_t y) { return boost_of(x) > boost_of(y); });
-            if ((int64_t)order.size() > start[c]) present += 1;
+            present += start[c + 1] > 0 ? 1 : 0;
+            start[c + 1] += start[c];
         }
-        start[10] = count;
         // One mask for the whole batch: every query has the same selectivity, nothing to order.
         const bool boosted = filt && o->mask_index;
         if (!which && present == 1 && !boosted)
             return launch(h, mode, filt, b, spill, max_slots, s, tier);
+        order.assign((size_t)count, 0);
+        {
+            std::vector<int64_t> next(start.begin(), start.end() - 1);
+            for (int64_t t = 0; t < count; ++t)
+                order[(size_t)next[cls[(size_t)t]]++] = (int32_t)(which ? (*which)[t] : t);
+        }
+        for (int c = 5; c < 10; ++c)  // filtered: most selective masks first (queue order)
+            std::stable_sort(order.begin() + start[c], order.begin() + start[c + 1],
+                             [&](int32_t x, int32_t y) { return boost_of(x) > boost_of(y); });
         HM_TRY(h->subset.ensure(count), "alloc");
         HM_TRY(hipMemcpyAsync(h->subset.p, order.data(), count * 4, hipMemcpyHostToDevice, s),
                "copy subsets");
-        std::vector<uint8_t> boost;
         if (boosted) {
-            boost.resize((size_t)count);
+            boost.assign((size_t)count, 0);
             for (int64_t t = 0; t < count; ++t) boost[(size_t)t] = boost_of(order[(size_t)t]);
             HM_TRY(h->boost.ensure(count), "alloc");
             HM_TRY(hipMemcpyAsync(h->boost.p, boost.data(), count, hipMemcpyHostToDevice, s),
@@ -538,9 +548,11 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
                 mx = std::max(mx, efv.empty() ? b.ef_all : efv[order[t]]);
             bc.max_ef = mx;
             const int st2 = launch(h, mode, filt && c >= 5, bc, spill, max_slots, s, tier);
-            if (st2) return st2;
+            if (st2) {
+                (void)hipStreamSynchronize(s);  // the copies from `order` and `boost` are done
+                return st2;
+            }
         }
-        HM_TRY(hipStreamSynchronize(s), "search");  // `order` must outlive its copy
         return HM_OK;
     };
     st = run(nullptr, std::min<int64_t>(h->n, kDefaultSpill), 0, 2);
