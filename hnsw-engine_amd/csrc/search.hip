@@ -631,9 +631,11 @@ HM_DEV void greedy_layer(const SearchArgs &a, const Lds &L, Visited &V, Query &Q
 
 // Selects a prefix of the spilled candidates (by (key, id)) of size 1..c_cap: returns the
 // boundary (bk, bi) such that the entries strictly below it form that prefix. Radix select over
-// the 96-bit composite (key, id), 8 bits at a time, most significant first.
+// the 96-bit composite (key, id), 8 bits at a time, most significant first. Entries with a key
+// above wk take no part; when at most cap entries are left the boundary is (kInfKey, kInfId),
+// which selects them all.
 HM_DEV void spill_boundary(const Lds &L, const uint64_t *sk, const uint32_t *si, int w, int cap,
-                           uint64_t &bk, uint32_t &bi) {
+                           uint64_t wk, uint64_t &bk, uint32_t &bi) {
     const int lane = lane_id();
     uint64_t pk = 0;
     uint32_t pi = 0;
@@ -645,6 +647,7 @@ HM_DEV void spill_boundary(const Lds &L, const uint64_t *sk, const uint32_t *si,
             if (j >= w) continue;
             const uint64_t k = sk[j];
             const uint32_t id = si[j];
+            if (k > wk) continue;
             bool match;
             if (t == 0) match = true;
             else if (t < 8) match = (k >> (64 - 8 * t)) == (pk >> (64 - 8 * t));
@@ -666,6 +669,11 @@ HM_DEV void spill_boundary(const Lds &L, const uint64_t *sk, const uint32_t *si,
             if (lane >= o) incl += v;
         }
         const int excl = incl - s;
+        if (t == 0 && shfl(incl, 63) <= cap) {  // every entry left fits
+            bk = kInfKey;
+            bi = kInfId;
+            return;
+        }
         // First bin whose inclusive count exceeds the capacity.
         int dstar = 256, before = 0;
         if (incl > cap && excl <= cap) {
@@ -748,7 +756,7 @@ HM_DEV bool layer0(const SearchArgs &a, const Lds &L, Visited &V, Query &Q, int 
             if (w == 0) break;
             uint64_t bk = kInfKey;
             uint32_t bi = kInfId;
-            if (w > a.c_cap) spill_boundary(L, sk, si, w, a.c_cap, bk, bi);
+            if (w > a.c_cap) spill_boundary(L, sk, si, w, a.c_cap, kInfKey, bk, bi);
             int nc = 0, nw = 0;
             uint64_t mk = kInfKey;
             uint32_t mi = kInfId;
@@ -1656,12 +1664,19 @@ HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
     };
     for (;;) {
         if (ch == cn && spn > 0) {
-            // C ran dry: prune the spill, then bring back its smallest kC entries.
+            // C ran dry: bring back the smallest kC spilled entries. With R full, entries beyond
+            // its worst key can never be expanded: the select ignores them and the pass that
+            // moves the selected entries to C drops them while it compacts the others in place
+            // (they move down, never past one not yet read: 4 x 64 entries are read before any
+            // of them is written).
             __threadfence();
             const uint64_t wk = worst_key();
-            // in place: the kept entries move down, never past one not yet read (4 x 64
-            // entries are read before any of them is written)
-            int w = 0;
+            uint64_t bk = kInfKey;
+            uint32_t bi = kInfId;
+            spill_boundary(L, sk, si, spn, kC, wk, bk, bi);
+            int nc = 0, nw = 0;
+            uint64_t mk = kInfKey;
+            uint32_t mi = kInfId;
             for (int b = 0; b < spn; b += 256) {
                 uint64_t k4[4];
                 uint32_t i4[4];
@@ -1673,64 +1688,35 @@ HM_DEV bool layer0_beam(const SearchArgs &a, const Lds &L, Visited &V, Query &Q,
                 }
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
-                    const bool kp = b + 64 * t + lane < spn && k4[t] <= wk;
-                    const uint64_t m = ballot(kp);
-                    if (kp) {
-                        sk[w + rank_below(m)] = k4[t];
-                        si[w + rank_below(m)] = i4[t];
+                    const bool v = b + 64 * t + lane < spn && k4[t] <= wk;
+                    const uint64_t k = k4[t];
+                    const uint32_t id = i4[t];
+                    const bool sel = v && less(k, id, bk, bi);
+                    const bool rest = v && !sel;
+                    const uint64_t m1 = ballot(sel), m2 = ballot(rest);
+                    if (sel) {
+                        L.ck_b[nc + rank_below(m1)] = k;
+                        L.ci_b[nc + rank_below(m1)] = id;
                     }
-                    w += popc(m);
+                    if (rest) {
+                        sk[nw + rank_below(m2)] = k;
+                        si[nw + rank_below(m2)] = id;
+                        if (less(k, id, mk, mi)) {
+                            mk = k;
+                            mi = id;
+                        }
+                    }
+                    nc += popc(m1);
+                    nw += popc(m2);
                 }
             }
             __threadfence();
-            spn = 0;
+            spn = nw;
             smin = E::inf();
             cn = ch = 0;
-            if (w > 0) {
-                uint64_t bk = kInfKey;
-                uint32_t bi = kInfId;
-                if (w > kC) spill_boundary(L, sk, si, w, kC, bk, bi);
-                int nc = 0, nw = 0;
-                uint64_t mk = kInfKey;
-                uint32_t mi = kInfId;
-                for (int b = 0; b < w; b += 256) {
-                    uint64_t k4[4];
-                    uint32_t i4[4];
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        const int j = b + 64 * t + lane;
-                        k4[t] = j < w ? sk[j] : kInfKey;
-                        i4[t] = j < w ? si[j] : kInfId;
-                    }
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        const bool v = b + 64 * t + lane < w;
-                        const uint64_t k = k4[t];
-                        const uint32_t id = i4[t];
-                        const bool sel = v && less(k, id, bk, bi);
-                        const bool rest = v && !sel;
-                        const uint64_t m1 = ballot(sel), m2 = ballot(rest);
-                        if (sel) {
-                            L.ck_b[nc + rank_below(m1)] = k;
-                            L.ci_b[nc + rank_below(m1)] = id;
-                        }
-                        if (rest) {
-                            sk[nw + rank_below(m2)] = k;
-                            si[nw + rank_below(m2)] = id;
-                            if (less(k, id, mk, mi)) {
-                                mk = k;
-                                mi = id;
-                            }
-                        }
-                        nc += popc(m1);
-                        nw += popc(m2);
-                    }
-                }
-                __threadfence();
-                if (nc == 0) return false;  // cannot happen: the select takes 1..kC entries
+            if (nc > 0) {  // none when every spilled entry lay beyond the worst key
                 wave_min_pair(mk, mi);
                 smin = nw > 0 ? E::make(mk, mi) : E::inf();
-                spn = nw;
                 int p2 = 1;
                 while (p2 < nc) p2 <<= 1;
                 bitonic_sort(L.ck_b, L.ci_b, nc, p2);
