@@ -168,3 +168,24 @@ def test_massive_ties_overflow_the_register_beam_and_rerun() -> None:
         want = oracle.search_batch(eng.native_arrays(), eng.entry_point, eng.max_level, "l2",
                                    q, k, ef, threads=8)
         _same(got, want, f"ties ef={ef}")
+
+
+def test_host_query_forms_on_an_8bit_index(built_u8) -> None:
+    """hm_search on an index of 8-bit rows: integer queries 0..255 travel as bytes (packed on the
+    host) and take the exact kernel; a batch with one fractional, negative or out-of-range value
+    takes the float64 kernel for all its queries. Both against the oracle, which computes the
+    float64 distance of whatever values it is given."""
+    eng, data, queries = built_u8
+    base = queries[:300].copy()
+    cases = {"integer": base}
+    for tag, (row, col, v) in {"fraction": (5, 3, 17.5), "negative": (140, 0, -1.0),
+                               "above_255": (299, 31, 256.0)}.items():
+        q = base.copy()
+        q[row, col] = v
+        cases[tag] = q
+    for tag, q in cases.items():
+        print(f"case host queries {tag}", flush=True)
+        got = eng.search_many(q, 10, 48)
+        want = oracle.search_batch(eng.native_arrays(), eng.entry_point, eng.max_level, "l2",
+                                   q, 10, 48, threads=4)
+        _same(got, want, tag)
