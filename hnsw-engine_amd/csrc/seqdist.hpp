@@ -16,15 +16,16 @@
 namespace hm {
 
 // Adds the elements of one 16-byte chunk (16 bytes of U8 codes or 4 floats), element i0 first,
-// the ones at dim and beyond skipped.
+// the ones at dim and beyond skipped. L2: (x - q)^2 terms, else x q terms (the metric is a
+// template argument so that each element computes only its own terms).
 // q[i] as float64, by a scalar load (the address is wave-uniform and the data read-only).
 template <class QT> HM_DEV double qval(const QT *q, int i) {
     using CP = const __attribute__((address_space(4))) QT *;
     return (double)((CP)q)[i];
 }
 
-template <bool U8, class QT>
-HM_DEV void seq_chunk(double &acc, uint4 w, const QT *q, int i0, int dim, bool l2) {
+template <bool U8, bool L2, class QT>
+HM_DEV void seq_chunk(double &acc, uint4 w, const QT *q, int i0, int dim) {
 #pragma clang fp contract(off)
     const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
     constexpr int kPer = U8 ? 16 : 4;
@@ -35,7 +36,7 @@ HM_DEV void seq_chunk(double &acc, uint4 w, const QT *q, int i0, int dim, bool l
         const double x = U8 ? (double)((ws[e >> 2] >> (8 * (e & 3))) & 255u)
                             : (double)__uint_as_float(ws[e]);
         const double qi = qval(q, i);
-        if (l2) {
+        if constexpr (L2) {
             const double d = x - qi;
             acc = acc + d * d;
         } else {
@@ -46,8 +47,8 @@ HM_DEV void seq_chunk(double &acc, uint4 w, const QT *q, int i0, int dim, bool l
 
 // seq_chunk for a chunk wholly inside the row: no per-element bounds checks, so the chunk's
 // query values load together (one wait) instead of one scalar round trip per element.
-template <bool U8, class QT>
-HM_DEV void seq_chunk_full(double &acc, uint4 w, const QT *q, int i0, bool l2) {
+template <bool U8, bool L2, class QT>
+HM_DEV void seq_chunk_full(double &acc, uint4 w, const QT *q, int i0) {
 #pragma clang fp contract(off)
     const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
     constexpr int kPer = U8 ? 16 : 4;
@@ -58,7 +59,7 @@ HM_DEV void seq_chunk_full(double &acc, uint4 w, const QT *q, int i0, bool l2) {
     for (int e = 0; e < kPer; ++e) {
         const double x = U8 ? (double)((ws[e >> 2] >> (8 * (e & 3))) & 255u)
                             : (double)__uint_as_float(ws[e]);
-        if (l2) {
+        if constexpr (L2) {
             const double d = x - qv[e];
             acc = acc + d * d;
         } else {
@@ -67,16 +68,16 @@ HM_DEV void seq_chunk_full(double &acc, uint4 w, const QT *q, int i0, bool l2) {
     }
 }
 
-template <bool U8, class QT>
-HM_DEV double seq_row(const void *row, const QT *q, int dim, int metric) {
+// The sum over one row for one metric family (L2: sum (x - q)^2; else sum x q).
+template <bool U8, bool L2, class QT>
+HM_DEV double seq_sum(const void *row, const QT *q, int dim) {
 #pragma clang fp contract(off)
-    const bool l2 = metric == kL2;
     double acc = 0.0;
     if (!U8 && (dim & 3) != 0) {  // float rows not 16-byte aligned: element by element
         const float *r = (const float *)row;
         for (int i = 0; i < dim; ++i) {
             const double qi = qval(q, i);
-            if (l2) {
+            if constexpr (L2) {
                 const double d = (double)r[i] - qi;
                 acc = acc + d * d;
             } else {
@@ -102,14 +103,24 @@ HM_DEV double seq_row(const void *row, const QT *q, int dim, int metric) {
 #pragma unroll
             for (int t = 0; t < kB; ++t) {
                 const int i0 = (b0 + t) * kPer;
-                if (i0 + kPer <= dim) seq_chunk_full<U8, QT>(acc, cur[t], q, i0, l2);
-                else if (i0 < dim) seq_chunk<U8, QT>(acc, cur[t], q, i0, dim, l2);
+                if (i0 + kPer <= dim) seq_chunk_full<U8, L2, QT>(acc, cur[t], q, i0);
+                else if (i0 < dim) seq_chunk<U8, L2, QT>(acc, cur[t], q, i0, dim);
             }
 #pragma unroll
             for (int t = 0; t < kB; ++t) cur[t] = nxt[t];
         }
     }
-    return l2 ? acc : (metric == kIP ? -acc : 1.0 - acc);
+    return acc;
+}
+
+// The metric is the same for every lane, so this branch is uniform: each lane runs the loop of
+// its metric only.
+template <bool U8, class QT>
+HM_DEV double seq_row(const void *row, const QT *q, int dim, int metric) {
+#pragma clang fp contract(off)
+    if (metric == kL2) return seq_sum<U8, true, QT>(row, q, dim);
+    const double acc = seq_sum<U8, false, QT>(row, q, dim);
+    return metric == kIP ? -acc : 1.0 - acc;
 }
 
 }  // namespace hm
