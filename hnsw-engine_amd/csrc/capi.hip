@@ -557,6 +557,13 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
     };
     st = run(nullptr, std::min<int64_t>(h->n, kDefaultSpill), 0, 2);
     if (st) return st;
+    {
+        const size_t out = (size_t)nq * (rerank_q ? rerank_k : o->kstride) * 8;
+        prefault_output(ids, out);
+        prefault_output(dists, out);
+        if (dcount) prefault_output(dcount, (size_t)nq * 8);
+        if (hops) prefault_output(hops, (size_t)nq * 8);
+    }
     std::vector<int32_t> failv(nq), dcv(nq), hv(nq);
     // The failure flags and everything the caller gets back are queued behind the search and
     // fetched with one synchronisation; a batch with failed queries (rare) runs them again and

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <sys/mman.h>
+#include <unistd.h>
 
 #include <string>
 
@@ -64,6 +66,20 @@ struct PinBuf {
         cap = 0;
     }
 };
+
+// Maps every page of a host output range ahead of the device-to-host copy that fills it. Called
+// while the kernels run: the pages of a fresh allocation otherwise fault in one at a time
+// inside the copy, after the kernels have ended (about 0.15 ms per MB of output). Only pages
+// wholly inside the range are populated (MADV_POPULATE_WRITE maps them and changes no data);
+// where that call is refused, one byte per such page is written (the copy overwrites it).
+inline void prefault_output(void *p, size_t bytes) {
+    static const uintptr_t page = (uintptr_t)sysconf(_SC_PAGESIZE);
+    if (!p || bytes < page) return;
+    uintptr_t a = ((uintptr_t)p + page - 1) & ~(page - 1);
+    const uintptr_t e = ((uintptr_t)p + bytes) & ~(page - 1);
+    if (e <= a || madvise((void *)a, e - a, MADV_POPULATE_WRITE) == 0) return;
+    for (; a < e; a += page) *(volatile uint8_t *)a = 0;
+}
 
 inline int ceil_log2(int64_t v) {
     int r = 0;

@@ -1088,6 +1088,8 @@ extern "C" int hm_exact(const float *data, int64_t n, int32_t dim, int32_t metri
         HM_TRY(bits.alloc(b.size()), "alloc");
         HM_TRY(hipMemcpy(bits.p, b.data(), b.size() * 4, hipMemcpyHostToDevice), "copy mask");
     }
+    prefault_output(ids, (size_t)nq * k * 8);
+    prefault_output(dists, (size_t)nq * k * 8);
     const int rc = exact_device(dx.p, n, dim, metric, dq.p, nq, k, allowed ? bits.p : nullptr, int8,
                                 oid.p, od.p, s, device);
     if (rc != HM_OK) return rc;
@@ -1156,6 +1158,8 @@ extern "C" int hm_index_exact(hm_index *h, const float *queries, int64_t nq, int
         HM_TRY(hipMemcpyAsync(bits.p, b.data(), b.size() * 4, hipMemcpyHostToDevice, h->stream),
                "copy mask");
     }
+    prefault_output(ids, (size_t)nq * k * 8);  // while the queries upload
+    prefault_output(dists, (size_t)nq * k * 8);
     const int rc = exact_handle(h, dq.p, nq, k, allowed ? bits.p : nullptr, int8, oid.p, od.p);
     if (rc != HM_OK) return rc;
     HM_TRY(hipMemcpyAsync(ids, oid.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, h->stream), "copy ids");

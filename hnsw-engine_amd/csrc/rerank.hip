@@ -102,6 +102,8 @@ extern "C" int hm_rerank(hm_index *h, const float *queries, int64_t nq, const in
                "copy");
     HM_TRY(rerank_device(h, h->q32.p, h->cand.p, nq, fetch, k, h->ids.p, h->dists.p, s),
            "rerank launch");
+    prefault_output(ids, (size_t)nq * k * 8);  // while the re-rank runs
+    prefault_output(dists, (size_t)nq * k * 8);
     HM_TRY(hipMemcpyAsync(ids, h->ids.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, s), "copy");
     HM_TRY(hipMemcpyAsync(dists, h->dists.p, (size_t)nq * k * 8, hipMemcpyDeviceToHost, s), "copy");
     HM_TRY(hipStreamSynchronize(s), "rerank");
