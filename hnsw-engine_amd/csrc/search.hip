@@ -96,11 +96,11 @@ HM_DEV Lds carve_fast(uint8_t *base, const SearchArgs &a, int mode) {
     uint8_t *p = base;
     L.qb = p;
     p += mode == kExactU8 ? up(a.dimp) : up(a.dim * 8);
-    L.rk[0] = (uint64_t *)p; p += 64 * 8 * a.nr;
+    L.rk[0] = (uint64_t *)p; p += 64 * 8 * a.nr + 16;
     L.ck[0] = (uint64_t *)p; p += 64 * 8;
     L.ri[0] = (uint32_t *)p; p += 64 * 4;
     L.ci[0] = (uint32_t *)p; p += 64 * 4;
-    L.nbal = (uint32_t *)p; p += 64 * 4 * a.nr;
+    L.nbal = (uint32_t *)p; p += 64 * 4 * a.nr + 16;
     // One expansion reads the new ids (nbi) and the partial sums (nbacc) before its merge
     // stages R in rk[0] / ri[0] / nbal, so they share that space.
     L.nbi = L.ri[0];
@@ -312,9 +312,9 @@ HM_DEV uint64_t dist_exact_rows128(const SearchArgs &a, const Lds &L, int nnew, 
         const int32_t got = __builtin_amdgcn_ds_bpermute(src, row);
         if ((lane >> 3) == it) s = got;
     }
-    uint64_t key = kInfKey;
-    if (lane < nnew) key = a.metric == kL2 ? key_of_exact(s + qq, kL2) : key_of_exact(-s, a.metric);
-    return key;
+    uint64_t key = a.metric == kL2 ? key_of_exact(s + qq, kL2) : key_of_exact(-s, a.metric);
+    __asm__ volatile("" : "+v"(key));  // computed by every lane: the select below stays a select
+    return lane < nnew ? key : kInfKey;
 }
 
 template <int SH>
@@ -1336,7 +1336,9 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
             S.last_new = 0;
             S.last_acc = 0;
         }
-        const int32_t nb = lane < 32 ? a.links0[(size_t)cid * 32 + lane] : -1;
+        // both half-waves read the row (the same lines), so the load needs no lane mask
+        const int32_t raw = a.links0[(size_t)cid * 32 + (lane & 31)];
+        const int32_t nb = lane < 32 ? raw : -1;
         visited_make_room(L, V, 32);
         const bool isnew = visit_lanes_quotient(L, V, (uint32_t)nb, nb >= 0);
         visited_note(V, isnew, (uint32_t)nb);
@@ -1344,11 +1346,15 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
         const int nnew = popc(nm);
         if (nnew == 0) goto expanded;
         {
-            if (isnew) L.nbi[rank_below(nm)] = (uint32_t)nb;
+            int rkn = rank_below(nm);
+            __asm__ volatile("" : "+v"(rkn));  // every lane computes: the store takes no lane mask
+            L.nbi[isnew ? rkn : 32 + (lane & 31)] = (uint32_t)nb;
             wave_sync();
             V.count += nnew;
             const uint64_t ki = distances<MODE, kShapeM16D128>(a, L, nnew, Q.qq, Q.qi);
-            const uint64_t ne = lane < nnew ? (ki << 32) | L.nbi[lane] : kInfEntry;
+            uint32_t nbl = L.nbi[lane];  // every lane reads (in bounds), so no lane mask
+            __asm__ volatile("" : "+v"(nbl));
+            const uint64_t ne = lane < nnew ? (ki << 32) | nbl : kInfEntry;
             const uint32_t nk = key_hi(ne);
             Q.dcount += nnew;
             if constexpr (AUX != 0) {
@@ -1407,22 +1413,24 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
 #pragma unroll
             for (int s = 0; s < NR; ++s) {
                 rflag[s] = (expd[s] >> lane) & 1ull;
-                if (in_r(s, rn) && pos_r[s] < ef) {
-                    L.rk[0][pos_r[s]] = re[s];
-                    L.nbal[pos_r[s]] = rflag[s] ? 1u : 0u;
-                }
+                const int pr = in_r(s, rn) && pos_r[s] < ef ? pos_r[s] : 64 * NR;  // spare slot
+                L.rk[0][pr] = re[s];
+                L.nbal[pr] = rflag[s] ? 1u : 0u;
             }
-            if (acc && pos_a < ef) {
-                L.rk[0][pos_a] = ne;
-                L.nbal[pos_a] = 0u;
+            {
+                const int pa = acc && pos_a < ef ? pos_a : 64 * NR;
+                L.rk[0][pa] = ne;
+                L.nbal[pa] = 0u;
             }
             wave_sync();
             uint64_t nre[NR];
             bool nflag[NR];
 #pragma unroll
             for (int s = 0; s < NR; ++s) {
-                nre[s] = in_r(s, nrn) ? L.rk[0][lane + 64 * s] : kInfEntry;
-                nflag[s] = in_r(s, nrn) && L.nbal[lane + 64 * s] != 0u;
+                const uint64_t e = L.rk[0][lane + 64 * s];  // every lane reads (in bounds)
+                const uint32_t f = L.nbal[lane + 64 * s];
+                nre[s] = in_r(s, nrn) ? e : kInfEntry;
+                nflag[s] = in_r(s, nrn) && f != 0u;
             }
             if (total > ef) {
                 // Evictions: keep the unexpanded ones whose key equals the new worst (an R entry

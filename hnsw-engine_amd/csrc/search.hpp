@@ -128,16 +128,17 @@ __host__ __device__ inline int visited_hash_bytes(int h_log2, int id_bits) {
 }
 
 // Bytes of LDS one wave of the register-beam kernel needs: query, (64 nr)-entry R scatter buffer
-// with its flags (shared with the new-neighbour staging), 64-entry tie list and the visited
-// hash (layer 0 with 1 << h_log2 slots, upper layers with at most 512). 128 dimensions, 1M ids
-// and 4,096 slots: 10,112 bytes (16 waves per CU).
+// with its flags (shared with the new-neighbour staging; each with a spare entry that takes the
+// stores of entries not kept, so those stores need no lane mask), 64-entry tie list and the
+// visited hash (layer 0 with 1 << h_log2 slots, upper layers with at most 512). 128 dimensions,
+// 1M ids and 4,096 slots: 10,144 bytes (16 waves per CU).
 inline int lds_bytes_fast(int mode, int dim, int dimp, int h_log2, int id_bits, int nr = 1) {
     auto up = [](int v) { return (v + 15) & ~15; };
     int q = mode == kExactU8 ? up(dimp) : up(dim * 8);
     const int upper = h_log2 < 9 ? h_log2 : 9;
     const int h0 = visited_hash_bytes(h_log2, id_bits), hu = visited_hash_bytes(upper, id_bits);
     const int hash = h0 > hu ? h0 : hu;
-    return q + nr * 64 * 8 + 64 * 4 + 64 * 8 + 64 * 4 + nr * 64 * 4 + hash;
+    return q + nr * 64 * 8 + 16 + 64 * 4 + 64 * 8 + 64 * 4 + nr * 64 * 4 + 16 + hash;
 }
 
 // Bytes of LDS one wave of the general register beam needs: query, new ids and row sums, R
