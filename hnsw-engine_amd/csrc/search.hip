@@ -291,27 +291,42 @@ HM_DEV void exact_rows_strided(const SearchArgs &a, const Lds &L, int nnew) {
 // Constant shape (128-byte rows): 8 lanes per row, 8 rows per pass, at most 4 passes; each
 // pass's row sums reach lanes 0..nnew-1 by ds_bpermute (lane i takes row i's sum from lane
 // 8 (i % 8) of pass i / 8), with no round trip through LDS memory.
-HM_DEV uint64_t dist_exact_rows128(const SearchArgs &a, const Lds &L, int nnew, int32_t qq) {
+// NP passes (the pass count is wave-uniform: one straight-line form per count), so that the
+// passes' sums interleave and their ds_bpermute results are awaited once.
+template <int NP>
+HM_DEV int32_t rows128_sums(const SearchArgs &a, const Lds &L, int nnew) {
     const int lane = lane_id();
     const int c = lane & 7;
     const uint4 q4 = ((const uint4 *)L.qb)[c];
-    uint32_t ids[4];
+    uint32_t ids[NP];
 #pragma unroll
-    for (int it = 0; it < 4; ++it) ids[it] = L.nbi[min(it * 8 + (lane >> 3), nnew - 1)];
-    uint4 xs[4];
+    for (int it = 0; it < NP; ++it) ids[it] = L.nbi[min(it * 8 + (lane >> 3), nnew - 1)];
+    uint4 xs[NP];
 #pragma unroll
-    for (int it = 0; it < 4; ++it) {
-        if (it * 8 < nnew) xs[it] = *(const uint4 *)(a.vec_u8 + ((size_t)ids[it] << 7) + c * 16);
-    }
+    for (int it = 0; it < NP; ++it)
+        xs[it] = *(const uint4 *)(a.vec_u8 + ((size_t)ids[it] << 7) + c * 16);
+    int32_t row[NP];
+#pragma unroll
+    for (int it = 0; it < NP; ++it) row[it] = exact_partial(xs[it], q4, a.metric);
+#pragma unroll
+    for (int it = 0; it < NP; ++it) row[it] = group_sum(row[it], 8);
     const int src = c << 5;  // byte address of lane 8 (lane % 8)
-    int32_t s = 0;
+    int32_t got[NP];
 #pragma unroll
-    for (int it = 0; it < 4; ++it) {
-        if (it * 8 >= nnew) break;
-        const int32_t row = group_sum(exact_partial(xs[it], q4, a.metric), 8);
-        const int32_t got = __builtin_amdgcn_ds_bpermute(src, row);
-        if ((lane >> 3) == it) s = got;
-    }
+    for (int it = 0; it < NP; ++it) got[it] = __builtin_amdgcn_ds_bpermute(src, row[it]);
+    int32_t s = got[0];
+#pragma unroll
+    for (int it = 1; it < NP; ++it) s = (lane >> 3) == it ? got[it] : s;
+    return s;
+}
+
+HM_DEV uint64_t dist_exact_rows128(const SearchArgs &a, const Lds &L, int nnew, int32_t qq) {
+    const int lane = lane_id();
+    const int np = (nnew + 7) >> 3;  // 1..4
+    const int32_t s = np == 1   ? rows128_sums<1>(a, L, nnew)
+                      : np == 2 ? rows128_sums<2>(a, L, nnew)
+                      : np == 3 ? rows128_sums<3>(a, L, nnew)
+                                : rows128_sums<4>(a, L, nnew);
     uint64_t key = a.metric == kL2 ? key_of_exact(s + qq, kL2) : key_of_exact(-s, a.metric);
     __asm__ volatile("" : "+v"(key));  // computed by every lane: the select below stays a select
     return lane < nnew ? key : kInfKey;
