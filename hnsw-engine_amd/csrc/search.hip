@@ -109,6 +109,23 @@ HM_DEV Lds carve_fast(uint8_t *base, const SearchArgs &a, int mode) {
     return L;
 }
 
+// carve_fast for the constant shape (128 query bytes, NR R entries per lane): every size is known
+// at compile time, so the LDS addresses become instruction offsets instead of registers.
+template <int NR> HM_DEV Lds carve_fast_m16(uint8_t *base) {
+    Lds L = {};
+    uint8_t *p = base;
+    L.qb = p; p += 128;
+    L.rk[0] = (uint64_t *)p; p += 64 * 8 * NR + 16;
+    L.ck[0] = (uint64_t *)p; p += 64 * 8;
+    L.ri[0] = (uint32_t *)p; p += 64 * 4;
+    L.ci[0] = (uint32_t *)p; p += 64 * 4;
+    L.nbal = (uint32_t *)p; p += 64 * 4 * NR + 16;
+    L.nbi = L.ri[0];
+    L.nbacc = (int32_t *)L.nbal;
+    L.hash = (uint32_t *)p;
+    return L;
+}
+
 // Layout of the general register beam (layer0_beam).
 HM_DEV Lds carve_beam(uint8_t *base, const SearchArgs &a, int mode) {
     auto up = [](int v) { return (v + 15) & ~15; };
@@ -2147,9 +2164,12 @@ template <int MODE, bool FILT, bool FAST, int AUX, int SH, int BNR = 0>
 HM_DEV void search_loop(const SearchArgs &a) {
     extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
     const int slot = blockIdx.x;
-    const Lds L = BNR > 0 ? carve_beam(lds_raw, a, MODE)
-                  : FAST  ? carve_fast(lds_raw, a, MODE)
-                          : carve(lds_raw, a, MODE);
+    Lds L;
+    if constexpr (BNR > 0) L = carve_beam(lds_raw, a, MODE);
+    else if constexpr (FAST && SH == kShapeM16D128 && MODE == kExactU8)
+        L = carve_fast_m16<1>(lds_raw);  // (the wider beams measured no gain from it)
+    else if constexpr (FAST) L = carve_fast(lds_raw, a, MODE);
+    else L = carve(lds_raw, a, MODE);
     Visited V;
     V.bits = a.bitsets + (size_t)slot * a.bitset_words;
     V.list = a.vlist + (size_t)slot * a.vlist_cap;
