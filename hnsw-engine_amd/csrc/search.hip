@@ -1420,7 +1420,8 @@ HM_DEV bool layer0_packed(const SearchArgs &a, const Lds &L, Visited &V, Query &
                 const int q = __ffsll((long long)m) - 1;
                 const uint64_t eq = readlane_u64(ne, q);
                 const uint32_t kq = (uint32_t)(eq >> 32);
-                int c = popc(lanes_le_u32(nk, kq) & part & ((1ull << q) - 1ull));
+                // part & ~m: the candidates before q (m still holds q and those after it)
+                int c = popc(lanes_le_u32(nk, kq) & part & ~m);
 #pragma unroll
                 for (int s = 0; s < NR; ++s) c += popc(lanes_le_u32(rkey[s], kq) & rvalid[s]);
                 if (c >= ef) continue;
