@@ -70,15 +70,15 @@ struct PinBuf {
 // Maps every page of a host output range ahead of the device-to-host copy that fills it. Called
 // while the kernels run: the pages of a fresh allocation otherwise fault in one at a time
 // inside the copy, after the kernels have ended (about 0.15 ms per MB of output). Only pages
-// wholly inside the range are populated (MADV_POPULATE_WRITE maps them and changes no data);
-// where that call is refused, one byte per such page is written (the copy overwrites it).
+// wholly inside the range are populated (MADV_POPULATE_WRITE maps them and changes no data). A
+// performance hint only: where the call is refused (an older kernel, memory that is not
+// ordinary host memory) nothing is done and the copy maps the pages itself.
 inline void prefault_output(void *p, size_t bytes) {
     static const uintptr_t page = (uintptr_t)sysconf(_SC_PAGESIZE);
     if (!p || bytes < page) return;
-    uintptr_t a = ((uintptr_t)p + page - 1) & ~(page - 1);
+    const uintptr_t a = ((uintptr_t)p + page - 1) & ~(page - 1);
     const uintptr_t e = ((uintptr_t)p + bytes) & ~(page - 1);
-    if (e <= a || madvise((void *)a, e - a, MADV_POPULATE_WRITE) == 0) return;
-    for (; a < e; a += page) *(volatile uint8_t *)a = 0;
+    if (e > a) (void)madvise((void *)a, e - a, MADV_POPULATE_WRITE);
 }
 
 inline int ceil_log2(int64_t v) {
