@@ -583,10 +583,9 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
                    "rerank launch");
             HM_TRY(hipMemcpyAsync(ids, h->rids.p, rn * 8, hipMemcpyDeviceToHost, s), "copy ids");
             HM_TRY(hipMemcpyAsync(dists, h->rdists.p, rn * 8, hipMemcpyDeviceToHost, s), "copy");
-        } else {
-            HM_TRY(hipMemcpyAsync(ids, h->ids.p, on * 8, hipMemcpyDeviceToHost, s), "copy ids");
-            HM_TRY(hipMemcpyAsync(dists, h->dists.p, on * 8, hipMemcpyDeviceToHost, s),
-                   "copy dists");
+        } else {  // large results through pinned staging and four host threads
+            HM_TRY(h->staging.copy_out(ids, h->ids.p, on * 8, s), "copy ids");
+            HM_TRY(h->staging.copy_out(dists, h->dists.p, on * 8, s), "copy dists");
         }
         HM_TRY(hipMemcpyAsync(dcv.data(), h->dc.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");
         HM_TRY(hipMemcpyAsync(hv.data(), h->hops.p, nq * 4, hipMemcpyDeviceToHost, s), "copy");

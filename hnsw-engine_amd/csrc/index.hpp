@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "hostcopy.hpp"
 
 struct hm_index {
     int device = 0;
@@ -64,6 +65,8 @@ struct hm_index {
     // hm_search_graph_dev: the captured launches of the last batch shape and what they bake in
     hipGraphExec_t graph_exec = nullptr;
     std::vector<int64_t> graph_key;
+    // large host results: pinned staging and copy threads (made on first use)
+    hm::Staging staging;
 
     void release_all() {
         hm::DevBuf<int32_t> *i32[] = {&links0, &links_up, &upper, &queue,    &kq,   &efq,
@@ -95,6 +98,7 @@ struct hm_index {
         rdists.release();
         stop_w.release();
         stats.release();
+        staging.release();
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         graph_exec = nullptr;
         graph_key.clear();
