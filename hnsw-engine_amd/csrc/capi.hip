@@ -805,7 +805,20 @@ int hm_search(hm_index *h, const float *queries, const hm_search_opts *o, int64_
         const size_t count = (size_t)o->nq * h->dim;
         HM_TRY(hipSetDevice(h->device), "hipSetDevice");
         HM_TRY(h->pin_q8.ensure(count), "pinned allocation");
-        if (pack_u8_integers(queries, count, h->pin_q8.p))
+        bool bytes = true;
+        if (count >= (size_t(1) << 18)) {  // the handle's copy threads pack a quarter each
+            CopyPool &team = h->staging.team();
+            std::vector<uint8_t> good(team.parts(), 1);
+            team.run([&](size_t p, size_t parts) {
+                const size_t per = ((count + parts - 1) / parts + 15) & ~size_t(15);
+                const size_t a = std::min(count, p * per), e = std::min(count, a + per);
+                good[p] = a == e || pack_u8_integers(queries + a, e - a, h->pin_q8.p + a);
+            });
+            for (size_t p = 0; p < team.parts(); ++p) bytes = bytes && good[p];
+        } else {
+            bytes = pack_u8_integers(queries, count, h->pin_q8.p);
+        }
+        if (bytes)
             return search_host(h, kExactU8, nullptr, nullptr, o, ids, dists, dcount, hops,
                                nullptr, 0, h->pin_q8.p);
     }

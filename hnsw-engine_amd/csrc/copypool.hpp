@@ -1,23 +1,25 @@
-// A few host threads that copy one buffer together with the calling thread (plain C++, no HIP:
-// the large-result copies of hostcopy.hpp use it, and a CPU test builds it alone).
+// A few host threads that work through one job together with the calling thread: a buffer copy
+// or any function of (part, parts) (plain C++, no HIP: the large-result copies and the query
+// packing of the host entry points use it, and a CPU test builds it alone).
 #pragma once
 #include <stdint.h>
 #include <string.h>
 
 #include <algorithm>
 #include <condition_variable>
+#include <functional>
 #include <mutex>
 #include <thread>
 #include <vector>
 
 namespace hm {
 
-// Owned by one index handle, whose calls do not overlap, so copy() is never entered twice at
-// once.
+// Owned by one index handle, whose calls do not overlap, so run() and copy() are never entered
+// twice at once.
 class CopyPool {
   public:
     explicit CopyPool(int workers) {
-        for (int i = 0; i < workers; ++i) th_.emplace_back([this, i] { run(i + 1); });
+        for (int i = 0; i < workers; ++i) th_.emplace_back([this, i] { run_worker(i + 1); });
     }
     ~CopyPool() {
         {
@@ -30,39 +32,46 @@ class CopyPool {
     CopyPool(const CopyPool &) = delete;
     CopyPool &operator=(const CopyPool &) = delete;
 
-    // dst[0, n) = src[0, n), split in (workers + 1) 64-byte aligned parts; returns when done.
-    void copy(void *dst, const void *src, size_t n) {
-        const size_t parts = th_.size() + 1;
+    size_t parts() const { return th_.size() + 1; }
+
+    // f(p, parts()) for every p in [0, parts()), part 0 on the calling thread; returns when all
+    // are done.
+    void run(const std::function<void(size_t, size_t)> &f) {
         {
             std::lock_guard<std::mutex> g(m_);
-            d_ = (char *)dst;
-            s_ = (const char *)src;
-            n_ = n;
-            per_ = ((n + parts - 1) / parts + 63) & ~size_t(63);  // covers n, never 0
+            job_ = &f;
             pending_ = (int)th_.size();
             ++gen_;
         }
         cv_.notify_all();
-        part(0);
+        f(0, parts());
         std::unique_lock<std::mutex> l(m_);
         done_.wait(l, [&] { return pending_ == 0; });
+        job_ = nullptr;
+    }
+
+    // dst[0, n) = src[0, n), split in parts() 64-byte aligned pieces.
+    void copy(void *dst, const void *src, size_t n) {
+        const size_t per = ((n + parts() - 1) / parts() + 63) & ~size_t(63);  // covers n, never 0
+        char *d = (char *)dst;
+        const char *s = (const char *)src;
+        run([&](size_t p, size_t) {
+            const size_t a = p * per;
+            if (a < n) memcpy(d + a, s + a, std::min(n, a + per) - a);
+        });
     }
 
   private:
-    void part(size_t p) {
-        const size_t a = p * per_;
-        if (a >= n_) return;
-        memcpy(d_ + a, s_ + a, std::min(n_, a + per_) - a);
-    }
-    void run(size_t p) {
+    void run_worker(size_t p) {
         uint64_t seen = 0;
         std::unique_lock<std::mutex> l(m_);
         for (;;) {
             cv_.wait(l, [&] { return stop_ || gen_ != seen; });
             if (stop_) return;
             seen = gen_;
+            const std::function<void(size_t, size_t)> *f = job_;
             l.unlock();
-            part(p);
+            (*f)(p, parts());
             l.lock();
             if (--pending_ == 0) done_.notify_one();
         }
@@ -74,9 +83,7 @@ class CopyPool {
     bool stop_ = false;
     uint64_t gen_ = 0;
     int pending_ = 0;
-    char *d_ = nullptr;
-    const char *s_ = nullptr;
-    size_t n_ = 0, per_ = 0;
+    const std::function<void(size_t, size_t)> *job_ = nullptr;
 };
 
 }  // namespace hm

@@ -25,11 +25,15 @@ struct Staging {
     hipEvent_t ev[2] = {nullptr, nullptr};
     std::unique_ptr<CopyPool> pool;
 
+    CopyPool &team() {
+        if (!pool) pool.reset(new CopyPool(3));
+        return *pool;
+    }
     hipError_t ready() {
         hipError_t e = buf.ensure(2 * kChunk);
         for (int i = 0; i < 2 && e == hipSuccess; ++i)
             if (!ev[i]) e = hipEventCreateWithFlags(&ev[i], hipEventDisableTiming);
-        if (e == hipSuccess && !pool) pool.reset(new CopyPool(3));
+        if (e == hipSuccess) team();
         return e;
     }
     void release() {

@@ -1,6 +1,7 @@
-"""The host copy threads behind large search results (csrc/copypool.hpp), built alone with g++
-and run on the CPU: every byte of copies of odd and chunk-sized lengths arrives, pools are
-created and joined repeatedly, and one pool serves many copies in a row."""
+"""The host threads behind large search results and query packing (csrc/copypool.hpp), built
+alone with g++ and run on the CPU: every byte of copies of odd and chunk-sized lengths arrives,
+run() gives every part exactly once, pools are created and joined repeatedly, and one pool
+serves many jobs in a row."""
 
 from __future__ import annotations
 
@@ -42,6 +43,20 @@ int main() {
                         }
                 }
             }
+        }
+    }
+    {  // run(): every part once, on its own index range
+        hm::CopyPool pool(3);
+        for (int rep = 0; rep < 100; ++rep) {
+            std::vector<int> hit(1000, 0);
+            pool.run([&](size_t p, size_t parts) {
+                for (size_t i = p; i < hit.size(); i += parts) hit[i] += 1;
+            });
+            for (int h : hit)
+                if (h != 1) {
+                    puts("run missed or repeated an index");
+                    return 1;
+                }
         }
     }
     for (int i = 0; i < 200; ++i) {  // create and join repeatedly

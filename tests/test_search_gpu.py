@@ -174,7 +174,8 @@ def test_host_query_forms_on_an_8bit_index(built_u8) -> None:
     """hm_search on an index of 8-bit rows: integer queries 0..255 travel as bytes (packed on the
     host) and take the exact kernel; a batch with one fractional, negative or out-of-range value
     takes the float64 kernel for all its queries. Both against the oracle, which computes the
-    float64 distance of whatever values it is given."""
+    float64 distance of whatever values it is given; also at 2^18 values, where the host packs
+    with four threads."""
     eng, data, queries = built_u8
     base = queries[:300].copy()
     cases = {"integer": base}
@@ -183,6 +184,13 @@ def test_host_query_forms_on_an_8bit_index(built_u8) -> None:
         q = base.copy()
         q[row, col] = v
         cases[tag] = q
+    # A batch of 2^18 values is checked and packed by four host threads (a quarter each): all
+    # integers, and one fraction in the last quarter.
+    big = np.tile(base, (28, 1))[:8192]
+    cases["parallel_integer"] = big
+    q = big.copy()
+    q[8000, 7] = 3.25
+    cases["parallel_fraction_last_quarter"] = q
     for tag, q in cases.items():
         print(f"case host queries {tag}", flush=True)
         got = eng.search_many(q, 10, 48)
