@@ -542,10 +542,12 @@ TilePlan plan_tile(int64_t m, int64_t nqb, int nch, int k) {
     p.chunk = round_up((m + p.nchunks - 1) / p.nchunks, kTileRows);
     p.nchunks = (int)((m + p.chunk - 1) / p.chunk);
     const int nsb = (int)(p.chunk / kTileRows);
-    // A quarter of each chunk (at least two superblocks): measured on 10,000 x 1M x 128, a
-    // sixteenth makes the bound pass 2x cheaper but the looser threshold costs more in the range
-    // pass (more 16-row groups reach it) and in the collect.
-    p.sample_every = std::max(1, std::min(4, nsb / 2));
+    // A sixth of each chunk (at least two superblocks): measured on 10,000 x 1M x 128 (bound,
+    // range and collect together), every 2nd / 3rd / 4th / 5th / 6th / 8th / 16th superblock
+    // 1.74 / 1.58 / 1.42-1.56 / 1.40 / 1.39-1.50 / 1.39-1.52 / 1.62 ms: a sparser sample makes
+    // the bound pass cheaper but its looser threshold costs more in the range pass (more 16-row
+    // groups reach it) and in the collect.
+    p.sample_every = std::max(1, std::min(6, nsb / 2));
     if (g_tile_sample.load() > 0) p.sample_every = std::min(g_tile_sample.load(), nsb);
     p.T = k <= 32 ? 4 : 8;
     // The threshold admits about k * sample_every rows per query (the k-th of a 1/sample_every
