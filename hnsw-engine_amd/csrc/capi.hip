@@ -161,8 +161,11 @@ __global__ void clear_words_kernel(uint32_t *p, int64_t words) {
         p[i] = 0u;
 }
 
+// The two-phase schedule's descent kernels: 16 upper links, up to 128 dimensions (one-byte
+// rows of exactly 128 bytes for the integer-key mode).
 bool two_phase_shape(const hm_index *h, int mode) {
-    return mode == kExactU8 && h->w1 == 16 && h->dimp == 128 && h->dim <= 128;
+    if (h->w1 != 16 || h->dim > 128) return false;
+    return mode == kExactU8 ? h->dimp == 128 : (mode == kSeqU8 || mode == kSeqF32);
 }
 
 // Launches one batch. fixed == null: scratch from the handle's buffers; otherwise every scratch
@@ -312,11 +315,13 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap,
     a.out_hops = b.hops;
     a.out_fail = b.fail;
     a.slots = slots;
-    // More queries than wave slots on the register beam (128 one-byte dimensions, 16 upper
-    // links): every query's upper-layer descent first, 16 lanes per query (descend_kernel), then
-    // layer 0 with the queries whose layer-0 entry lies farthest from them first, which
-    // shortens the launch's tail. Measured on 10,000 queries over 1M x 128 against one launch:
-    // ef 32 / 48 / 64 / 96 -12 / -17 / -16 / -14 %, ids identical.
+    // More queries than wave slots on the register beam (16 upper links, up to 128
+    // dimensions): every query's upper-layer descent first, 16 lanes per query (descend_kernel;
+    // descend_seq_kernel for the float64 distances), then layer 0 with the queries whose
+    // layer-0 entry lies farthest from them first, which shortens the launch's tail. Measured on
+    // 10,000 queries over 1M x 128 against one launch: integer queries ef 32 / 48 / 64 / 96
+    // -12 / -17 / -16 / -14 %; fractional queries ef 16 / 64 / 128 -22 / -22 / -17 %; all
+    // outputs identical.
     if ((a.fast || a.beam) && two_phase_shape(h, mode) && !a.trace && b.nq > slots &&
         h->tune_schedule == 0) {
         a.descent_cap = h->tune_descent_cap > 0 ? h->tune_descent_cap : kDescentMaxCap;

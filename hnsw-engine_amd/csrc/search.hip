@@ -2238,8 +2238,9 @@ __global__ void reset_queue_kernel(int32_t *q) { q[0] = 0; }
 // enters layer 0 from its nearest neighbours, the more expansions it tends to take, and taking
 // long queries first shortens a launch's tail. With `relative` (inner product, whose keys are
 // 2^31 minus the dot product and so share one log bucket) the key's excess over the batch's
-// smallest key is bucketed instead. One workgroup of 1024 threads; it also resets the work
-// queue of the layer-0 launch that follows.
+// smallest key is bucketed instead; relative = 2 (float64 keys, key_of_double): that excess is
+// a 64-bit number, bucketed by its log2 with 2 fraction bits. One workgroup of 1024 threads; it
+// also resets the work queue of the layer-0 launch that follows.
 __global__ __launch_bounds__(1024) void order_by_entry_kernel(const uint64_t *key, int nq,
                                                               int relative,
                                                               const uint8_t *boost,
@@ -2266,8 +2267,15 @@ __global__ __launch_bounds__(1024) void order_by_entry_kernel(const uint64_t *ke
         __syncthreads();
     }
     const uint64_t base = kmin;
-    auto bucket = [base](uint64_t k0) {
+    const bool wide = relative == 2;
+    auto bucket = [base, wide](uint64_t k0) {
         const uint64_t k = k0 - base;
+        if (wide) {
+            if (k == 0) return 255;
+            const int e = 63 - __clzll((long long)k);
+            const int frac = e >= 2 ? (int)((k >> (e - 2)) & 3u) : (int)((k << (2 - e)) & 3u);
+            return 255 - min(255, e * 4 + frac);
+        }
         const uint32_t v = (uint32_t)min<uint64_t>(k, 0xFFFFFFFEull) + 1u;  // >= 1
         const int e = 31 - __clz(v);
         const int frac = e >= 3 ? (int)((v >> (e - 3)) & 7u) : (int)((v << (3 - e)) & 7u);
@@ -2442,6 +2450,112 @@ __global__ __launch_bounds__(256) void descend_kernel(SearchArgs a) {
     }
 }
 
+// descend_kernel for the float64 sequential distances (codes with float64 queries: U8; float32
+// rows with float32 queries: !U8): the same schedule, rule and counters, 16 lanes per query, lane
+// j of a query computing link j's distance with seq_row in the reference's order, from the
+// query staged in LDS as float64 (a float32 value converts exactly); keys are key_of_double.
+// Up to 128 dimensions.
+template <bool U8>
+__global__ __launch_bounds__(256) void descend_seq_kernel(SearchArgs a) {
+    __shared__ __attribute__((aligned(16))) uint32_t table[kDescentGroups][kDescentSlots];
+    __shared__ __attribute__((aligned(16))) double qlds[kDescentGroups][128];
+    const int lane = lane_id();
+    const int j = lane & 15;
+    const int g = threadIdx.x >> 4;
+    const int first = lane & 48;  // this query's first lane in the wave
+    const int pi = blockIdx.x * kDescentGroups + g;  // position in the launch's schedule arrays
+    const bool act = pi < a.nq;
+    const int pc = act ? pi : a.nq - 1;  // groups past the batch compute on a valid query
+    const int qc = a.subset ? a.subset[pc] : pc;
+    uint32_t *const tab = table[g];
+    const double *const qd = qlds[g];
+    for (int i = j; i < a.dim; i += 16)
+        qlds[g][i] = U8 ? a.q_f64[(size_t)qc * a.dim + i] : (double)a.q_f32[(size_t)qc * a.dim + i];
+    wave_sync();
+    auto dist = [&](uint32_t id) __attribute__((always_inline)) -> uint64_t {
+        const double d = U8 ? seq_row<true, double, false>(a.vec_u8 + (size_t)id * a.dimp, qd,
+                                                            a.dim, a.metric)
+                            : seq_row<false, double, false>(a.vec_f32 + (size_t)id * a.dim, qd,
+                                                             a.dim, a.metric);
+        return key_of_double(d);
+    };
+    // Entry distance (counted, numba_engine.py:449-451): the group's first lane, then shared.
+    const uint32_t entry = (uint32_t)a.entry;
+    uint64_t cur_k = j == 0 ? dist(entry) : 0ull;
+    cur_k = shfl_u64(cur_k, first);
+    uint32_t cur_id = entry;
+    int32_t uoff = a.max_level > 0 ? a.upper_off[entry] : 0;
+    int dcount = 1, hops = 0;
+    bool flagged = false;
+    if (act) {
+        for (int layer = a.max_level; layer > 0 && !flagged; --layer) {
+#pragma unroll
+            for (int s = 0; s < kDescentSlots / 64; ++s)
+                *(uint4 *)&tab[64 * s + 4 * j] = make_uint4(0u, 0u, 0u, 0u);
+            wave_sync();
+            if (j == 0) tab[(cur_id * kHashMul) >> 24] = cur_id + 1u;
+            wave_sync();
+            int count = 1;
+            for (;;) {
+                if (count + 16 > a.descent_cap) {
+                    flagged = true;
+                    break;
+                }
+                hops += 1;
+                const int32_t nb = a.links_up[((int64_t)uoff + layer - 1) * 16 + j];
+                bool isnew = false;
+                if (nb >= 0) {  // the ids of a row are distinct
+                    const uint32_t v = (uint32_t)nb + 1u;
+                    uint32_t pos = ((uint32_t)nb * kHashMul) >> 24;
+                    for (;;) {
+                        const uint32_t old = atomicCAS(&tab[pos], 0u, v);
+                        if (old == 0u) {
+                            isnew = true;
+                            break;
+                        }
+                        if (old == v) break;
+                        pos = (pos + 1u) & (kDescentSlots - 1);
+                    }
+                }
+                const uint32_t gm = (uint32_t)(ballot(isnew) >> first) & 0xFFFFu;
+                count += __popc(gm);
+                dcount += __popc(gm);
+                uint64_t k64 = kInfKey;
+                int32_t nuo = 0;
+                if (isnew) {
+                    nuo = a.upper_off[nb];
+                    k64 = dist((uint32_t)nb);
+                }
+                // Minimum over the query's 16 lanes.
+                uint64_t m = k64;
+#pragma unroll
+                for (int o = 1; o < 16; o <<= 1) {
+                    const uint64_t other = shfl_u64(m, first | ((lane ^ o) & 15));
+                    m = other < m ? other : m;
+                }
+                const uint32_t hit = (uint32_t)(ballot(isnew && k64 == m) >> first) & 0xFFFFu;
+                const int src = first + (hit ? __ffs(hit) - 1 : 0);
+                const uint32_t nid = (uint32_t)shfl(nb, src);
+                const int32_t noff = shfl(nuo, src);
+                if (!(m < cur_k)) break;
+                cur_k = m;
+                cur_id = nid;
+                uoff = noff;
+            }
+            wave_sync();
+        }
+    }
+    if (act && j == 0) {
+        a.pre_key[pi] = flagged ? kInfKey : cur_k;
+        a.pre_id[pi] = cur_id;
+        a.pre_dc[pi] = dcount;
+        a.pre_hops[pi] = flagged ? -1 : hops;
+    }
+}
+
+template __global__ void descend_seq_kernel<true>(SearchArgs);
+template __global__ void descend_seq_kernel<false>(SearchArgs);
+
 template __global__ void search_kernel<kExactU8, false>(SearchArgs);
 template __global__ void search_kernel<kExactU8, true>(SearchArgs);
 template __global__ void search_kernel<kSeqF32, false>(SearchArgs);
@@ -2553,14 +2667,16 @@ hipError_t search_launch(int mode, bool filt, const SearchArgs &a, int blocks, h
         return hipGetLastError();
     }
     // Two phases: every descent (16 lanes per query), the order, then layer 0 longest first.
-    if (a.nq < 1 || a.w1 != 16 || a.dimp != 128 || a.dim > 128 || mode != kExactU8 ||
+    if (a.nq < 1 || a.w1 != 16 || a.dim > 128 || (mode == kExactU8 && a.dimp != 128) ||
         a.descent_cap < 1 || a.descent_cap > kDescentMaxCap)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(descend_kernel, dim3((a.nq + kDescentGroups - 1) / kDescentGroups),
-                       dim3(256), 0, s, a);
+    const dim3 dgrid((a.nq + kDescentGroups - 1) / kDescentGroups);
+    if (mode == kExactU8) hipLaunchKernelGGL(descend_kernel, dgrid, dim3(256), 0, s, a);
+    else if (mode == kSeqU8) hipLaunchKernelGGL(descend_seq_kernel<true>, dgrid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(descend_seq_kernel<false>, dgrid, dim3(256), 0, s, a);
     hipLaunchKernelGGL(order_by_entry_kernel, dim3(1), dim3(1024), 0, s, a.pre_key, a.nq,
-                       a.metric == kIP ? 1 : 0, a.order_boost, const_cast<int32_t *>(a.order),
-                       a.queue);
+                       mode != kExactU8 ? 2 : a.metric == kIP ? 1 : 0, a.order_boost,
+                       const_cast<int32_t *>(a.order), a.queue);
     SearchArgs l = a;
     l.phase = 2;
     hipLaunchKernelGGL(fn, dim3(blocks), dim3(64), a.lds_per_wave, s, l);

@@ -49,7 +49,8 @@ struct SearchArgs {
     int32_t *out_dc;
     int32_t *out_hops;
     int32_t *out_fail;  // 1: candidate spill overflowed, rerun with a larger spill
-    // Two-phase schedule of the register beam: descend_kernel runs every query's upper-layer
+    // Two-phase schedule of the register beam: descend_kernel (descend_seq_kernel for the
+    // float64 distances) runs every query's upper-layer
     // descent and stores its layer-0 entry (key, id) and counters (pre_hops = -1: not done,
     // the layer-0 launch descends itself); phase 2 starts layer 0 from them, taking queries in
     // `order` (longest expected first). phase 0: one launch does both.

@@ -4,8 +4,9 @@
 // in flight while the current ones are summed, so a row of up to 128 bytes costs one memory
 // latency, and a longer one about one per 128 bytes rather than one per chunk. The query (dim
 // float64 or float32 values in global memory, the same for every lane) is read through the
-// scalar cache into scalar registers, off the chain of dependent adds; a float32 query value
-// converts to float64 exactly, as the reference's does.
+// scalar cache into scalar registers, off the chain of dependent adds (SQ = false: read by
+// ordinary loads, for a query in LDS or one per group of lanes); a float32 query value converts
+// to float64 exactly, as the reference's does.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -19,12 +20,17 @@ namespace hm {
 // the ones at dim and beyond skipped. L2: (x - q)^2 terms, else x q terms (the metric is a
 // template argument so that each element computes only its own terms).
 // q[i] as float64, by a scalar load (the address is wave-uniform and the data read-only).
-template <class QT> HM_DEV double qval(const QT *q, int i) {
-    using CP = const __attribute__((address_space(4))) QT *;
-    return (double)((CP)q)[i];
+// SQ false: the query lies in LDS or differs between lanes (an ordinary load).
+template <class QT, bool SQ = true> HM_DEV double qval(const QT *q, int i) {
+    if constexpr (SQ) {
+        using CP = const __attribute__((address_space(4))) QT *;
+        return (double)((CP)q)[i];
+    } else {
+        return (double)q[i];
+    }
 }
 
-template <bool U8, bool L2, class QT>
+template <bool U8, bool L2, class QT, bool SQ = true>
 HM_DEV void seq_chunk(double &acc, uint4 w, const QT *q, int i0, int dim) {
 #pragma clang fp contract(off)
     const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
@@ -35,7 +41,7 @@ HM_DEV void seq_chunk(double &acc, uint4 w, const QT *q, int i0, int dim) {
         if (i >= dim) break;
         const double x = U8 ? (double)((ws[e >> 2] >> (8 * (e & 3))) & 255u)
                             : (double)__uint_as_float(ws[e]);
-        const double qi = qval(q, i);
+        const double qi = qval<QT, SQ>(q, i);
         if constexpr (L2) {
             const double d = x - qi;
             acc = acc + d * d;
@@ -47,14 +53,14 @@ HM_DEV void seq_chunk(double &acc, uint4 w, const QT *q, int i0, int dim) {
 
 // seq_chunk for a chunk wholly inside the row: no per-element bounds checks, so the chunk's
 // query values load together (one wait) instead of one scalar round trip per element.
-template <bool U8, bool L2, class QT>
+template <bool U8, bool L2, class QT, bool SQ = true>
 HM_DEV void seq_chunk_full(double &acc, uint4 w, const QT *q, int i0) {
 #pragma clang fp contract(off)
     const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
     constexpr int kPer = U8 ? 16 : 4;
     double qv[kPer];
 #pragma unroll
-    for (int e = 0; e < kPer; ++e) qv[e] = qval(q, i0 + e);
+    for (int e = 0; e < kPer; ++e) qv[e] = qval<QT, SQ>(q, i0 + e);
 #pragma unroll
     for (int e = 0; e < kPer; ++e) {
         const double x = U8 ? (double)((ws[e >> 2] >> (8 * (e & 3))) & 255u)
@@ -69,14 +75,14 @@ HM_DEV void seq_chunk_full(double &acc, uint4 w, const QT *q, int i0) {
 }
 
 // The sum over one row for one metric family (L2: sum (x - q)^2; else sum x q).
-template <bool U8, bool L2, class QT>
+template <bool U8, bool L2, class QT, bool SQ = true>
 HM_DEV double seq_sum(const void *row, const QT *q, int dim) {
 #pragma clang fp contract(off)
     double acc = 0.0;
     if (!U8 && (dim & 3) != 0) {  // float rows not 16-byte aligned: element by element
         const float *r = (const float *)row;
         for (int i = 0; i < dim; ++i) {
-            const double qi = qval(q, i);
+            const double qi = qval<QT, SQ>(q, i);
             if constexpr (L2) {
                 const double d = (double)r[i] - qi;
                 acc = acc + d * d;
@@ -103,8 +109,8 @@ HM_DEV double seq_sum(const void *row, const QT *q, int dim) {
 #pragma unroll
             for (int t = 0; t < kB; ++t) {
                 const int i0 = (b0 + t) * kPer;
-                if (i0 + kPer <= dim) seq_chunk_full<U8, L2, QT>(acc, cur[t], q, i0);
-                else if (i0 < dim) seq_chunk<U8, L2, QT>(acc, cur[t], q, i0, dim);
+                if (i0 + kPer <= dim) seq_chunk_full<U8, L2, QT, SQ>(acc, cur[t], q, i0);
+                else if (i0 < dim) seq_chunk<U8, L2, QT, SQ>(acc, cur[t], q, i0, dim);
             }
 #pragma unroll
             for (int t = 0; t < kB; ++t) cur[t] = nxt[t];
@@ -115,11 +121,11 @@ HM_DEV double seq_sum(const void *row, const QT *q, int dim) {
 
 // The metric is the same for every lane, so this branch is uniform: each lane runs the loop of
 // its metric only.
-template <bool U8, class QT>
+template <bool U8, class QT, bool SQ = true>
 HM_DEV double seq_row(const void *row, const QT *q, int dim, int metric) {
 #pragma clang fp contract(off)
-    if (metric == kL2) return seq_sum<U8, true, QT>(row, q, dim);
-    const double acc = seq_sum<U8, false, QT>(row, q, dim);
+    if (metric == kL2) return seq_sum<U8, true, QT, SQ>(row, q, dim);
+    const double acc = seq_sum<U8, false, QT, SQ>(row, q, dim);
     return metric == kIP ? -acc : 1.0 - acc;
 }
 

@@ -190,6 +190,54 @@ def test_two_phase_batch_beyond_the_order_kernels_lds(request, which: str, metri
     _same(got, want, f"{which} 17000 queries")
 
 
+@pytest.mark.parametrize("which,metric", [("m16d128", "l2"), ("m16d128_ip", "ip")])
+@pytest.mark.parametrize("ef", [32, 100])
+def test_two_phase_fractional_queries_match_oracle(request, which: str, metric: str,
+                                                   ef: int) -> None:
+    """Fractional queries take the float64 distance (sequential sums): with fewer wave slots
+    than queries their descents run 16 lanes per query with the query in LDS, and the order
+    buckets the float64 keys by their excess over the batch's smallest. 397 queries, and 17,000
+    (the order kernel's recomputed buckets)."""
+    eng, data, queries = request.getfixturevalue(which)
+    rng = np.random.default_rng(61)
+    q = (queries[:397] + rng.uniform(-0.5, 0.5, (397, 128))).astype(np.float32)
+    rows = data[rng.integers(0, data.shape[0], size=17_000)]
+    big = (rows + rng.uniform(-20.0, 20.0, size=rows.shape)).astype(np.float32)
+    cases = [("397", q)] + ([("17000", big)] if ef == 32 else [])
+    for name, qq in cases:
+        want = _oracle(eng, qq, 10, ef, metric)
+        print(f"case {which} fractional two-phase ef={ef} {name} queries", flush=True)
+        try:
+            _tune(eng, waves_per_cu=1)
+            got = eng.search_many(qq, 10, ef)
+        finally:
+            _tune(eng)
+        _same(got, want, f"{which} fractional ef={ef} {name}")
+
+
+@pytest.mark.parametrize("ef", [16, 64])
+def test_two_phase_codes_search_matches_oracle(m16d128, ef: int) -> None:
+    """The codes traversal (uint8 codes, float64 code-space queries) with fewer wave slots than
+    queries: batched float64 descents, then layer 0 in the order of their keys."""
+    eng, data, queries = m16d128
+    rng = np.random.default_rng(67)
+    codes = data.astype(np.uint8)  # the vectors' own integer values as codes
+    qc = queries[:397].astype(np.float64) + rng.uniform(-0.5, 0.5, (397, 128))
+    want = oracle.search_batch(eng.native_arrays(), eng.entry_point, eng.max_level, "l2", qc,
+                               ef, ef, codes=codes, threads=8)
+    print(f"case m16d128 codes two-phase ef={ef}", flush=True)
+    d0, h0 = eng.distance_computations, eng.hops
+    try:
+        _tune(eng, waves_per_cu=1)
+        ids, d = eng.batch_search_codes(codes, qc, ef, ef)
+    finally:
+        _tune(eng)
+    np.testing.assert_array_equal(ids, want[0])
+    np.testing.assert_array_equal(d, want[1])
+    assert eng.distance_computations - d0 == int(want[2].sum())
+    assert eng.hops - h0 == int(want[3].sum())
+
+
 @pytest.mark.parametrize("ef", [48, 100])
 def test_padded_rows_match_oracle(m16d120, ef: int) -> None:
     """120 dimensions on the constant shape, in one launch and after the batched descent."""
