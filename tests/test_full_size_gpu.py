@@ -11,6 +11,9 @@ skip, so a run without the cache cannot pass silently).
 * the mixed batch of the secondary measurement (100,000 queries, ef 16-512, k 1/10/100, half
   masked at 5-50 %) run as one batch; 2,000 sampled queries equal the oracle's results for
   their own arguments, counters included;
+* fractional queries (float64 distances): all 10,000 queries moved by uniform offsets in
+  (-0.5, 0.5), ef 16 and 64, in one batch (batched float64 descents, then layer 0 longest
+  first) equal to the same batch in one launch, and 200 of them equal to the oracle;
 * exact oracle: 64 queries over all 1M rows, k = 10, ids and distances bit-identical to the C
   oracle's exact top-k; all 10,000 queries: results sorted by (distance, id), and every
   returned distance equal to an int64 recomputation of that row's distance.
@@ -49,6 +52,30 @@ def test_search_matches_oracle_at_full_size(full, ef: int) -> None:
                                q, 10, ef, threads=8)
     for g, w_, name in zip(got, want, ("ids", "dists", "dcount", "hops")):
         np.testing.assert_array_equal(g, w_, err_msg=f"ef={ef} {name}")
+
+
+@pytest.mark.parametrize("ef", [16, 64])
+def test_fractional_queries_at_full_size(full, ef: int) -> None:
+    from hnsw_amd import _native
+
+    eng, data, queries = full
+    rng = np.random.default_rng(71)
+    q = (queries + rng.uniform(-0.5, 0.5, queries.shape)).astype(np.float32)
+    print(f"case 1M fractional ef={ef}", flush=True)
+    got = eng.search_many(q, 10, ef)
+    lib = _native.lib()
+    try:
+        _native.check(lib.hm_set_tuning(eng._device(), 0, 0, 0, 0, 1, 0))
+        one = eng.search_many(q, 10, ef)
+    finally:
+        _native.check(lib.hm_set_tuning(eng._device(), 0, 0, 0, 0, 0, 0))
+    for g, o, name in zip(got, one, ("ids", "dists", "dcount", "hops")):
+        np.testing.assert_array_equal(g, o, err_msg=f"ef={ef} one launch {name}")
+    pick = rng.choice(q.shape[0], size=200, replace=False)
+    want = oracle.search_batch(eng.native_arrays(), eng.entry_point, eng.max_level, "l2",
+                               q[pick], 10, ef, threads=8)
+    for g, w_, name in zip(got, want, ("ids", "dists", "dcount", "hops")):
+        np.testing.assert_array_equal(g[pick], w_, err_msg=f"ef={ef} {name}")
 
 
 def test_exact_matches_oracle_at_full_size(full) -> None:
