@@ -59,6 +59,7 @@ struct Batch {
     double *dists = nullptr;
     int32_t *dc = nullptr, *hops = nullptr, *fail = nullptr;
     bool learned = false;  // use the handle's stop model (register-beam path only)
+    bool u8rows = false;   // kSeqU8 over the index's exact 8-bit rows (not codes)
     const int32_t *subset = nullptr;  // device: queue position -> query index (a kernel class)
     const uint8_t *boost = nullptr;   // device, per position: order boost (filtered classes)
     // traces (register-beam path only)
@@ -163,6 +164,13 @@ __global__ void clear_words_kernel(uint32_t *p, int64_t words) {
 
 // The two-phase schedule's descent kernels: 16 upper links, up to 128 dimensions (one-byte
 // rows of exactly 128 bytes for the integer-key mode).
+// Widens n float32 values to float64 (exact).
+__global__ void widen_kernel(const float *x, int64_t n, double *out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (double)x[i];
+}
+
 bool two_phase_shape(const hm_index *h, int mode) {
     if (h->w1 != 16 || h->dim > 128) return false;
     return mode == kExactU8 ? h->dimp == 128 : (mode == kSeqU8 || mode == kSeqF32);
@@ -176,6 +184,21 @@ bool two_phase_shape(const hm_index *h, int mode) {
 int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap, int max_slots,
            hipStream_t s, int tier = 2, Carver *fixed = nullptr) {
     Batch b = b0;
+    // Float32 queries on an index of 8-bit integers: the float64 sum reads the rows' exact
+    // one-byte copy (128 B per 128-dimension row instead of 512 B) with the queries widened to
+    // float64. Every term has the same value as over the float32 rows, so every sum, key and
+    // result is the same bit for bit. Not for traces, the stop model, a class subset (queries
+    // beyond nq) or the allocation-free scratch.
+    if (mode == kSeqF32 && h->u8.p && b.q32 && !fixed && !b.trace && !b.learned && !b.subset &&
+        b.nq > 0) {
+        const int64_t vals = b.nq * (int64_t)h->dim;
+        HM_TRY(h->qwide.ensure((size_t)vals), "query allocation");
+        const int grid = (int)std::min<int64_t>(2048, (vals + 255) / 256);
+        hipLaunchKernelGGL(widen_kernel, dim3(grid), dim3(256), 0, s, b.q32, vals, h->qwide.p);
+        b.q64 = h->qwide.p;
+        b.u8rows = true;
+        mode = kSeqU8;
+    }
     SearchArgs a;
     memset(&a, 0, sizeof(a));
     a.links0 = h->links0.p;
@@ -189,7 +212,7 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap,
     a.entry = (int)h->entry;
     a.max_level = h->max_level;
     a.vec_f32 = h->vec.p;
-    a.vec_u8 = mode == kExactU8 ? h->u8.p : h->codes.p;
+    a.vec_u8 = mode == kExactU8 || b.u8rows ? h->u8.p : h->codes.p;
     a.dim = h->dim;
     a.dimp = h->dimp;
     a.metric = h->metric;
@@ -213,7 +236,7 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap,
     // kernel.
     const int nr_ef = beam_width(b.max_ef);
     const bool allow_fast = tier >= 2;
-    const bool fast = allow_fast && !filt && b.max_ef <= 64 && mode != kSeqU8;
+    const bool fast = allow_fast && !filt && b.max_ef <= 64 && (mode != kSeqU8 || b.u8rows);
     const int wide_hlog = h->tune_fast_hlog ? h->tune_fast_hlog : 12;
     const bool wide = allow_fast && !filt && !fast && !b.trace && mode == kExactU8 && nr_ef > 0 &&
                       h->w0 == 32 && h->w1 == 16 && h->dimp == 128 &&

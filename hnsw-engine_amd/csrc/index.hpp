@@ -42,6 +42,7 @@ struct hm_index {
     // staging of the host-buffer entry points
     hm::DevBuf<float> q32;
     hm::DevBuf<double> q64;
+    hm::DevBuf<double> qwide;  // float32 queries as float64 for the float64 sum over u8 rows
     hm::DevBuf<int32_t> kq, efq, midx, dc, hops, fail, subset;
     hm::DevBuf<uint8_t> boost;  // per position of a mixed batch: order boosts
     // 8-bit queries of hm_search (exact mode): packed on the host into pinned memory, then copied
@@ -87,6 +88,7 @@ struct hm_index {
         spill_id.release();
         q32.release();
         q64.release();
+        qwide.release();
         q8.release();
         pin_q8.release();
         boost.release();
