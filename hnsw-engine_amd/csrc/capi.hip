@@ -187,10 +187,11 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap,
     // Float32 queries on an index of 8-bit integers: the float64 sum reads the rows' exact
     // one-byte copy (128 B per 128-dimension row instead of 512 B) with the queries widened to
     // float64. Every term has the same value as over the float32 rows, so every sum, key and
-    // result is the same bit for bit. Not for traces, the stop model, a class subset (queries
-    // beyond nq) or the allocation-free scratch.
-    if (mode == kSeqF32 && h->u8.p && b.q32 && !fixed && !b.trace && !b.learned && !b.subset &&
-        b.nq > 0) {
+    // result is the same bit for bit. Not for filtered batches (their kernels exist for the
+    // float32 rows only), traces, the stop model, a class subset (queries beyond nq) or the
+    // allocation-free scratch.
+    if (mode == kSeqF32 && h->u8.p && b.q32 && !filt && !fixed && !b.trace && !b.learned &&
+        !b.subset && b.nq > 0) {
         const int64_t vals = b.nq * (int64_t)h->dim;
         HM_TRY(h->qwide.ensure((size_t)vals), "query allocation");
         const int grid = (int)std::min<int64_t>(2048, (vals + 255) / 256);

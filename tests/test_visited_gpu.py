@@ -15,7 +15,7 @@ import pytest
 import oracle
 from hnsw_amd import _native, learned
 from hnsw_amd.engine import MI355XEngine
-from hnsw_amd.synthetic import clustered_u8
+from hnsw_amd.synthetic import clustered_u8, random_mask
 
 pytestmark = pytest.mark.gpu
 
@@ -213,6 +213,28 @@ def test_two_phase_fractional_queries_match_oracle(request, which: str, metric: 
         finally:
             _tune(eng)
         _same(got, want, f"{which} fractional ef={ef} {name}")
+
+
+def test_fractional_query_routes_match_oracle(m16d128) -> None:
+    """Fractional queries on the 8-bit index take three routes: unfiltered batches sum over the
+    rows' one-byte copy with the queries widened to float64; a masked batch and a batch with
+    per-query k and ef (one launch per kernel class) sum over the float32 rows. All give the
+    oracle's results."""
+    eng, data, queries = m16d128
+    rng = np.random.default_rng(73)
+    q = (queries[:300] + rng.uniform(-0.5, 0.5, (300, 128))).astype(np.float32)
+    mask = random_mask(data.shape[0], 0.3, 5)
+    print("case m16d128 fractional unfiltered / masked", flush=True)
+    for ef in (40, 200):
+        _same(eng.search_many(q, 10, ef), _oracle(eng, q, 10, ef), f"fractional ef={ef}")
+        got = eng.search_many(q, 10, ef, mask_pool=mask, mask_index=0)
+        want = oracle.search_batch(eng.native_arrays(), eng.entry_point, eng.max_level, "l2",
+                                   q, 10, ef, mask=mask, threads=8)
+        _same(got, want, f"fractional masked ef={ef}")
+    print("case m16d128 fractional per-query k / ef", flush=True)
+    k = rng.choice([1, 10, 50], size=300)
+    ef = np.maximum(rng.choice([16, 48, 100, 300], size=300), k)
+    _same(eng.search_many(q, k, ef), _oracle(eng, q, k, ef), "fractional per-query")
 
 
 @pytest.mark.parametrize("ef", [16, 64])
