@@ -61,6 +61,7 @@ struct Batch {
     bool learned = false;  // use the handle's stop model (register-beam path only)
     bool u8rows = false;   // kSeqU8 over the index's exact 8-bit rows (not codes)
     const int32_t *subset = nullptr;  // device: queue position -> query index (a kernel class)
+    int64_t q_rows = 0;               // with a subset: the query rows it indexes
     const uint8_t *boost = nullptr;   // device, per position: order boost (filtered classes)
     // traces (register-beam path only)
     float *trace = nullptr;
@@ -188,11 +189,12 @@ int launch(hm_index *h, int mode, bool filt, const Batch &b0, int64_t spill_cap,
     // one-byte copy (128 B per 128-dimension row instead of 512 B) with the queries widened to
     // float64. Every term has the same value as over the float32 rows, so every sum, key and
     // result is the same bit for bit. Not for filtered batches (their kernels exist for the
-    // float32 rows only), traces, the stop model, a class subset (queries beyond nq) or the
-    // allocation-free scratch.
+    // float32 rows only), traces, the stop model or the allocation-free scratch. A class subset
+    // widens every query row it indexes.
+    const int64_t q_rows = b.subset ? b.q_rows : b.nq;
     if (mode == kSeqF32 && h->u8.p && b.q32 && !filt && !fixed && !b.trace && !b.learned &&
-        !b.subset && b.nq > 0) {
-        const int64_t vals = b.nq * (int64_t)h->dim;
+        q_rows > 0) {
+        const int64_t vals = q_rows * (int64_t)h->dim;
         HM_TRY(h->qwide.ensure((size_t)vals), "query allocation");
         const int grid = (int)std::min<int64_t>(2048, (vals + 255) / 256);
         hipLaunchKernelGGL(widen_kernel, dim3(grid), dim3(256), 0, s, b.q32, vals, h->qwide.p);
@@ -571,6 +573,7 @@ int search_host(hm_index *h, int mode, const float *q32, const double *q64,
             Batch bc = b;
             bc.nq = cnt;
             bc.subset = h->subset.p + start[c];
+            bc.q_rows = count;
             bc.boost = boosted && c >= 5 ? h->boost.p + start[c] : nullptr;
             int mx = 0;
             for (int64_t t = start[c]; t < start[c + 1]; ++t)

@@ -216,10 +216,10 @@ def test_two_phase_fractional_queries_match_oracle(request, which: str, metric: 
 
 
 def test_fractional_query_routes_match_oracle(m16d128) -> None:
-    """Fractional queries on the 8-bit index take three routes: unfiltered batches sum over the
-    rows' one-byte copy with the queries widened to float64; a masked batch and a batch with
-    per-query k and ef (one launch per kernel class) sum over the float32 rows. All give the
-    oracle's results."""
+    """Fractional queries on the 8-bit index: unfiltered batches sum over the rows' one-byte copy
+    with the queries widened to float64 (also a batch with per-query k and ef, one launch per
+    kernel class over a subset of the widened rows); a masked batch sums over the float32 rows.
+    All give the oracle's results."""
     eng, data, queries = m16d128
     rng = np.random.default_rng(73)
     q = (queries[:300] + rng.uniform(-0.5, 0.5, (300, 128))).astype(np.float32)
