@@ -248,11 +248,44 @@ __global__ __launch_bounds__(kTileWaves * 64) void exact_tile_kernel(TileArgs a)
 }
 
 // V for each query: the k-th largest of its per_query sampled block scores (one wave per
-// query; bisection on the value with a wave-wide count).
+// query; bisection on the value with a wave-wide count). Up to 1024 scores stay in registers
+// (16 per lane), so a bisection step is 16 compares and a wave sum with no memory access; more
+// go through LDS.
 __global__ __launch_bounds__(64) void exact_threshold_kernel(const int32_t *bnd, int per_query,
                                                              int k, int32_t *trig) {
     extern __shared__ __attribute__((aligned(16))) int32_t vals[];
     const int q = blockIdx.x, lane = lane_id();
+    if (per_query <= 1024) {
+        int32_t v[16];
+        int32_t lo = INT_MAX, hi = INT_MIN;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int i = lane + 64 * j;
+            v[j] = i < per_query ? bnd[(size_t)q * per_query + i] : INT_MIN;
+            if (i < per_query) {
+                lo = min(lo, v[j]);
+                hi = max(hi, v[j]);
+            }
+        }
+        for (int m = 32; m >= 1; m >>= 1) {
+            lo = min(lo, shfl_xor(lo, m));
+            hi = max(hi, shfl_xor(hi, m));
+        }
+        // Largest x with #{v >= x} >= k; #{v >= lo} = per_query >= k. Padding (INT_MIN) is
+        // counted only at x = INT_MIN, where the answer is lo = INT_MIN anyway.
+        int64_t L = lo, H = hi;
+        while (L < H) {
+            const int64_t mid = L + (H - L + 1) / 2;
+            int c = 0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) c += (int64_t)v[j] >= mid ? 1 : 0;
+            c = wave_sum(c);
+            if (c >= k) L = mid;
+            else H = mid - 1;
+        }
+        if (lane == 0) trig[q] = (int32_t)L;
+        return;
+    }
     int32_t lo = INT_MAX, hi = INT_MIN;
     for (int i = lane; i < per_query; i += 64) {
         const int32_t v = bnd[(size_t)q * per_query + i];
@@ -515,8 +548,8 @@ hipError_t tile_launch(int mode, int nch, int qb, int T, const TileArgs &a, dim3
 
 hipError_t threshold_launch(const int32_t *bnd, int per_query, int k, int32_t *trig, int nq,
                             hipStream_t s) {
-    hipLaunchKernelGGL(exact_threshold_kernel, dim3(nq), dim3(64), per_query * 4, s, bnd,
-                       per_query, k, trig);
+    hipLaunchKernelGGL(exact_threshold_kernel, dim3(nq), dim3(64),
+                       per_query <= 1024 ? 0 : per_query * 4, s, bnd, per_query, k, trig);
     return hipGetLastError();
 }
 
