@@ -192,6 +192,63 @@ def codes(log=print, efs=(128,)) -> dict:
             "rows": rows}
 
 
+def fractional(log=print, efs=(16, 46, 64, 128)) -> dict:
+    """Fractional queries (float64 distances, the reference's sequential sum) on clustered_1m:
+    the 10,000 queries each moved by uniform offsets in (-0.5, 0.5) (seed 1), k = 10, queries
+    and outputs in HBM (``hm_search_dev``), median of 5 batches. On this index of 8-bit integers
+    the sum reads the rows' one-byte copy, so the algorithmic bytes are 128 per distance and 128
+    per expansion (one layer-0 links row)."""
+    import ctypes
+
+    import torch
+
+    from . import _native
+    from .workloads import WORKLOADS, load_or_build
+
+    w = WORKLOADS["clustered_1m"]
+    eng, data, queries = load_or_build(w, log=log)
+    lib, h = _native.lib(), eng._device()
+    dev = torch.device("cuda:0")
+    nq, k = queries.shape[0], 10
+    qf = queries + np.random.default_rng(1).uniform(-0.5, 0.5, queries.shape).astype(np.float32)
+    q = torch.from_numpy(qf).to(dev)
+    ids = torch.empty((nq, k), dtype=torch.int64, device=dev)
+    dists = torch.empty((nq, k), dtype=torch.float64, device=dev)
+    dc = torch.empty(nq, dtype=torch.int32, device=dev)
+    hops = torch.empty(nq, dtype=torch.int32, device=dev)
+    fail = torch.zeros(nq, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    rows = []
+    for ef in efs:
+        opts = _native.SearchOpts()
+        opts.nq, opts.k_all, opts.ef_all, opts.kstride, opts.mask_all = nq, k, ef, k, -1
+
+        def go():
+            _native.check(lib.hm_search_dev(h, p(q), ctypes.byref(opts), None, 0, p(ids),
+                                            p(dists), p(dc), p(hops), p(fail),
+                                            ctypes.c_void_p(stream.cuda_stream)))
+
+        go()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(5):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            go()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        t = sorted(ts)[2]
+        dq = dc.double().mean().item()
+        hq = hops.double().mean().item()
+        row = {"ef": ef, "qps": nq / t, "ms": t * 1e3, "mean_distances": dq, "mean_hops": hq,
+               "roofline": _hbm(dq * 128 + hq * 128, nq / t)}
+        rows.append(row)
+        log(f"[fractional] {json.dumps(row)}")
+    return {"workload": f"{w.name}: {w.n} x {w.dim}, {nq} fractional queries, k = 10",
+            "rows": rows}
+
+
 def mixed_batch(w, nq: int = 100_000):
     """The mixed batch on workload ``w`` (fixed seeds): ``(queries, k, ef, pool, mask_index)``
     with ef drawn from {16, 32, 64, 128, 256, 512} (raised to k), k from {1, 10, 100}, and half
@@ -231,11 +288,13 @@ def mixed(log=print) -> dict:
 def main(argv: list[str]) -> int:
     which = argv[1] if len(argv) > 1 else "filtered"
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-    runs = {"filtered": filtered, "quantized": quantized, "mixed": mixed, "codes": codes}
+    runs = {"filtered": filtered, "quantized": quantized, "mixed": mixed, "codes": codes,
+            "fractional": fractional}
     if which not in runs:
         print(f"unknown workload {which!r}; one of {sorted(runs)}", file=sys.stderr)
         return 2
-    extra = {"efs": tuple(int(v) for v in argv[2:])} if which == "codes" and argv[2:] else {}
+    extra = ({"efs": tuple(int(v) for v in argv[2:])}
+             if which in ("codes", "fractional") and argv[2:] else {})
     print(json.dumps(runs[which](log=log, **extra)), flush=True)
     return 0
 
